@@ -1,0 +1,546 @@
+// Deterministic cloud-in-cell deposition (deposit_particles,
+// open_spacecharge_mod.f90:156-288; harness twin depose_rho_scalar,
+// test_mod.f90:410-474).
+//
+//   1. cell key per particle: the upstream FP64 index formula, no contraction
+//   2. stable LSD radix sort of (cell, particle) pairs (wave-ballot ranking)
+//   3. per-cell partial sums of the 8 corner weights in particle order
+//      (one thread per light cell, one workgroup tree per heavy cell)
+//   4. each node sums the <= 8 cell/corner partials that touch it, in a fixed
+//      order.
+// No floating-point atomics: the result is bit-identical run to run.
+#include "osc_internal.h"
+
+namespace osc {
+
+namespace {
+constexpr int kThreads = 256;
+inline int64_t nblk(int64_t n) {
+  int64_t b = (n + kThreads - 1) / kThreads;
+  return b > 65536 * 8 ? 65536 * 8 : b;
+}
+
+// ---------------------------------------------------------------------------
+// reductions (fixed partition and tree: deterministic)
+// ---------------------------------------------------------------------------
+constexpr int kRedBlocks = 1024;
+
+__global__ void k_bbox_partial(const double* __restrict__ x, const double* __restrict__ y,
+                               const double* __restrict__ z, int64_t n, double* __restrict__ part) {
+  __shared__ double r[6][kThreads];
+  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * kThreads) {
+    v[0] = fmin(v[0], x[i]);
+    v[1] = fmin(v[1], y[i]);
+    v[2] = fmin(v[2], z[i]);
+    v[3] = fmax(v[3], x[i]);
+    v[4] = fmax(v[4], y[i]);
+    v[5] = fmax(v[5], z[i]);
+  }
+  for (int k = 0; k < 6; ++k) r[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 6; ++k)
+        r[k][threadIdx.x] = k < 3 ? fmin(r[k][threadIdx.x], r[k][threadIdx.x + s])
+                                  : fmax(r[k][threadIdx.x], r[k][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = r[threadIdx.x][0];
+}
+
+__global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double r[6][kThreads];
+  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int b = threadIdx.x; b < nb; b += kThreads)
+    for (int k = 0; k < 6; ++k) v[k] = k < 3 ? fmin(v[k], part[b * 6 + k]) : fmax(v[k], part[b * 6 + k]);
+  for (int k = 0; k < 6; ++k) r[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 6; ++k)
+        r[k][threadIdx.x] = k < 3 ? fmin(r[k][threadIdx.x], r[k][threadIdx.x + s])
+                                  : fmax(r[k][threadIdx.x], r[k][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[threadIdx.x] = r[threadIdx.x][0];
+}
+
+__global__ void k_sum_partial(const double* __restrict__ v, int64_t n, double* __restrict__ part) {
+  __shared__ double r[kThreads];
+  // contiguous chunk per block, strided within the block: fixed order
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < n ? b0 + chunk : n;
+  double s = 0.0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += kThreads) s = s + v[i];
+  r[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = kThreads / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) r[threadIdx.x] = r[threadIdx.x] + r[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = r[0];
+}
+
+__global__ void k_sum_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double r[kThreads];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kThreads) s = s + part[b];
+  r[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = kThreads / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) r[threadIdx.x] = r[threadIdx.x] + r[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = r[0];
+}
+
+// ---------------------------------------------------------------------------
+// step 1: cell keys
+// ---------------------------------------------------------------------------
+struct KeyParams {
+  double mn[3], dinv[3], base[3];
+  int nlo[3], nn[3];
+  uint32_t sentinel;
+};
+
+__device__ __forceinline__ bool cell_of(double x, double mn, double dinv, double base, int nlo,
+                                        int nn, int& i0) {
+  // ip = floor((x - min)*(1/dx) + 1) (open_spacecharge_mod.f90:259); node ip-nlo
+  const double f = floor((x - mn) * dinv + base);
+  const double lo = double(nlo), hi = double(nlo + nn - 2);
+  if (!(f >= lo && f <= hi)) return false;  // all 8 nodes must be on the mesh
+  i0 = int(f) - nlo;
+  return true;
+}
+
+__global__ void k_cic_keys(const double* __restrict__ x, const double* __restrict__ y,
+                           const double* __restrict__ z, int64_t n, KeyParams p,
+                           uint32_t* __restrict__ keys, uint32_t* __restrict__ ndrop) {
+  __shared__ uint32_t cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  uint32_t mine = 0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int i0, j0, k0;
+    bool ok = cell_of(x[i], p.mn[0], p.dinv[0], p.base[0], p.nlo[0], p.nn[0], i0) &&
+              cell_of(y[i], p.mn[1], p.dinv[1], p.base[1], p.nlo[1], p.nn[1], j0) &&
+              cell_of(z[i], p.mn[2], p.dinv[2], p.base[2], p.nlo[2], p.nn[2], k0);
+    keys[i] = ok ? uint32_t(i0) + uint32_t(p.nn[0]) * (uint32_t(j0) + uint32_t(p.nn[1]) * uint32_t(k0))
+                 : p.sentinel;
+    mine += ok ? 0u : 1u;
+  }
+  if (mine) atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(ndrop, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// step 2: stable LSD radix sort of (key, value) pairs
+// ---------------------------------------------------------------------------
+constexpr int RS_T = 256, RS_I = 16, RS_TILE = RS_T * RS_I, RS_W = RS_T / 64;
+
+__global__ void __launch_bounds__(RS_T)
+    k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift, uint32_t mask,
+              uint32_t* __restrict__ hist, int nb) {
+  __shared__ uint32_t h[256];
+  for (int i = threadIdx.x; i < 256; i += RS_T) h[i] = 0;
+  __syncthreads();
+  const int64_t base = int64_t(blockIdx.x) * RS_TILE;
+  for (int it = 0; it < RS_I; ++it) {
+    const int64_t i = base + it * RS_T + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d <= int(mask); d += RS_T) hist[int64_t(d) * nb + blockIdx.x] = h[d];
+}
+
+__global__ void __launch_bounds__(RS_T)
+    k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                 uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n, int shift,
+                 int bits, const uint32_t* __restrict__ offs, int nb) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wc[RS_W][256];
+  const uint32_t mask = (1u << bits) - 1u;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int d = threadIdx.x; d < 256; d += RS_T)
+    run[d] = d <= int(mask) ? offs[int64_t(d) * nb + blockIdx.x] : 0u;
+  const int64_t base = int64_t(blockIdx.x) * RS_TILE;
+  for (int it = 0; it < RS_I; ++it) {
+    for (int j = threadIdx.x; j < RS_W * 256; j += RS_T) (&wc[0][0])[j] = 0u;
+    __syncthreads();
+    const int64_t i = base + it * RS_T + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t k = valid ? kin[i] : 0u;
+    const uint32_t v = valid ? (vin ? vin[i] : uint32_t(i)) : 0u;
+    const uint32_t d = (k >> shift) & mask;
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < bits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    const uint32_t rank = __popcll(peers & lt);
+    if (valid && rank == 0) wc[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[d] + rank;
+      for (int w = 0; w < wave; ++w) pos += wc[w][d];
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+    for (int dd = threadIdx.x; dd <= int(mask); dd += RS_T) {
+      uint32_t s = 0;
+      for (int w = 0; w < RS_W; ++w) s += wc[w][dd];
+      run[dd] += s;
+    }
+    __syncthreads();
+  }
+}
+
+// exclusive scan of u32 (3-phase, recursive on block totals)
+constexpr int SC_T = 256, SC_I = 16, SC_TILE = SC_T * SC_I;
+
+__global__ void __launch_bounds__(SC_T)
+    k_scan_block(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t n,
+                 uint32_t* __restrict__ tot) {
+  __shared__ uint32_t s[SC_T];
+  const int64_t base = int64_t(blockIdx.x) * SC_TILE + int64_t(threadIdx.x) * SC_I;
+  uint32_t v[SC_I];
+  uint32_t acc = 0;
+  for (int k = 0; k < SC_I; ++k) {
+    const int64_t i = base + k;
+    v[k] = i < n ? in[i] : 0u;
+    const uint32_t t = v[k];
+    v[k] = acc;
+    acc += t;
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 1; off < SC_T; off <<= 1) {
+    uint32_t a = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += a;
+    __syncthreads();
+  }
+  const uint32_t excl = threadIdx.x ? s[threadIdx.x - 1] : 0u;
+  for (int k = 0; k < SC_I; ++k) {
+    const int64_t i = base + k;
+    if (i < n) out[i] = v[k] + excl;
+  }
+  if (threadIdx.x == SC_T - 1 && tot) tot[blockIdx.x] = s[SC_T - 1];
+}
+
+__global__ void k_scan_add(uint32_t* __restrict__ out, int64_t n,
+                           const uint32_t* __restrict__ tot) {
+  const int64_t i = blockIdx.x * int64_t(SC_TILE) + threadIdx.x;
+  const uint32_t a = tot[blockIdx.x];
+  for (int k = 0; k < SC_I; ++k) {
+    const int64_t j = i + int64_t(k) * SC_T;
+    if (j < n) out[j] += a;
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// step 3: cell ranges and per-cell corner sums
+// ---------------------------------------------------------------------------
+__global__ void k_cell_ranges(const uint32_t* __restrict__ ks, int64_t n,
+                              const uint32_t* __restrict__ ndrop, uint32_t* __restrict__ start,
+                              uint32_t* __restrict__ end) {
+  const int64_t nv = n - int64_t(*ndrop);
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nv;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t k = ks[i];
+    if (i == 0 || ks[i - 1] != k) start[k] = uint32_t(i);
+    if (i == nv - 1 || ks[i + 1] != k) end[k] = uint32_t(i + 1);
+  }
+}
+
+struct CellParams {
+  double mn[3], d[3], dinv[3];
+  int wofs[3];
+  int nn[3];
+  int hasq;      // per-particle charges
+  int harness;   // unit weights (scale applied at the node)
+  double q_each;
+};
+
+// the 8 CIC terms of one particle, corner c = di + 2 dj + 4 dk, each in the
+// upstream product order (open_spacecharge_mod.f90:274-281)
+__device__ __forceinline__ void cic_terms(double x, double y, double z, double q,
+                                          const CellParams& p, int i0, int j0, int k0,
+                                          double t[8]) {
+  const double ab = ((p.mn[0] - x) + double(i0 + p.wofs[0]) * p.d[0]) * p.dinv[0];
+  const double de = ((p.mn[1] - y) + double(j0 + p.wofs[1]) * p.d[1]) * p.dinv[1];
+  const double gh = ((p.mn[2] - z) + double(k0 + p.wofs[2]) * p.d[2]) * p.dinv[2];
+  if (p.harness) {
+    t[0] = ab * de * gh;
+    t[2] = ab * (1. - de) * gh;
+    t[6] = ab * (1. - de) * (1. - gh);
+    t[4] = ab * de * (1. - gh);
+    t[5] = (1. - ab) * de * (1. - gh);
+    t[7] = (1. - ab) * (1. - de) * (1. - gh);
+    t[3] = (1. - ab) * (1. - de) * gh;
+    t[1] = (1. - ab) * de * gh;
+  } else {
+    t[0] = ab * de * gh * q;
+    t[2] = ab * (1. - de) * gh * q;
+    t[6] = ab * (1. - de) * (1. - gh) * q;
+    t[4] = ab * de * (1. - gh) * q;
+    t[5] = (1. - ab) * de * (1. - gh) * q;
+    t[7] = (1. - ab) * (1. - de) * (1. - gh) * q;
+    t[3] = (1. - ab) * (1. - de) * gh * q;
+    t[1] = (1. - ab) * de * gh * q;
+  }
+}
+
+constexpr uint32_t kLight = 48;
+
+__global__ void k_cell_reduce(const uint32_t* __restrict__ sv, const uint32_t* __restrict__ start,
+                              const uint32_t* __restrict__ end, const double* __restrict__ x,
+                              const double* __restrict__ y, const double* __restrict__ z,
+                              const double* __restrict__ q, CellParams p, double* __restrict__ S,
+                              uint32_t* __restrict__ heavy, uint32_t* __restrict__ nheavy) {
+  const int64_t ncell = int64_t(p.nn[0]) * p.nn[1] * p.nn[2];
+  for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < ncell;
+       c += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t st = start[c], en = end[c];
+    if (en - st > kLight) {
+      heavy[atomicAdd(nheavy, 1u)] = uint32_t(c);
+      continue;
+    }
+    const int i0 = int(c % p.nn[0]), j0 = int((c / p.nn[0]) % p.nn[1]),
+              k0 = int(c / (int64_t(p.nn[0]) * p.nn[1]));
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t e = st; e < en; ++e) {
+      const uint32_t pi = sv[e];
+      double t[8];
+      cic_terms(x[pi], y[pi], z[pi], p.hasq ? q[pi] : p.q_each, p, i0, j0, k0, t);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s[k] = s[k] + t[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) S[k * ncell + c] = s[k];
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_cell_heavy(const uint32_t* __restrict__ sv, const uint32_t* __restrict__ start,
+                 const uint32_t* __restrict__ end, const double* __restrict__ x,
+                 const double* __restrict__ y, const double* __restrict__ z,
+                 const double* __restrict__ q, CellParams p, double* __restrict__ S,
+                 const uint32_t* __restrict__ heavy, const uint32_t* __restrict__ nheavy) {
+  __shared__ double red[8][kThreads];
+  const int64_t ncell = int64_t(p.nn[0]) * p.nn[1] * p.nn[2];
+  const uint32_t nh = *nheavy;
+  for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+    const uint32_t c = heavy[h];
+    const uint32_t st = start[c], en = end[c];
+    const int i0 = int(c % p.nn[0]), j0 = int((c / p.nn[0]) % p.nn[1]),
+              k0 = int(c / (uint32_t(p.nn[0]) * p.nn[1]));
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t e = st + threadIdx.x; e < en; e += kThreads) {
+      const uint32_t pi = sv[e];
+      double t[8];
+      cic_terms(x[pi], y[pi], z[pi], p.hasq ? q[pi] : p.q_each, p, i0, j0, k0, t);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s[k] = s[k] + t[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+      if (int(threadIdx.x) < w)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = red[k][threadIdx.x] + red[k][threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x < 8) S[threadIdx.x * ncell + c] = red[threadIdx.x][0];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// step 4: nodes
+// ---------------------------------------------------------------------------
+__global__ void k_node(const double* __restrict__ S, int nx, int ny, int nz, int scale_after,
+                       double scale, double* __restrict__ rho) {
+  const int64_t ncell = int64_t(nx) * ny * nz;
+  for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < ncell;
+       c += int64_t(gridDim.x) * blockDim.x) {
+    const int i = int(c % nx), j = int((c / nx) % ny), k = int(c / (int64_t(nx) * ny));
+    double acc = 0.0;
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc) {
+      const int ci = i - (cc & 1), cj = j - ((cc >> 1) & 1), ck = k - (cc >> 2);
+      if (ci >= 0 && cj >= 0 && ck >= 0) acc = acc + S[cc * ncell + ci + int64_t(nx) * (cj + int64_t(ny) * ck)];
+    }
+    rho[c] = scale_after ? acc * scale : acc;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+void bbox(osc_ctx* c, int64_t n, const double* x, const double* y, const double* z, double lo[3],
+          double hi[3]) {
+  DevBuf part(c->pool, sizeof(double) * 6 * kRedBlocks + 64);
+  double* pp = part.as<double>();
+  int nb = int(std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + kThreads - 1) / kThreads)));
+  hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(kThreads), 0, c->stream, x, y, z, n, pp);
+  OSC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kThreads), 0, c->stream, pp, nb,
+                     pp + 6 * kRedBlocks);
+  OSC_HIP(hipGetLastError());
+  double h[6];
+  OSC_HIP(hipMemcpyAsync(h, pp + 6 * kRedBlocks, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  OSC_HIP(hipStreamSynchronize(c->stream));
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = h[a];
+    hi[a] = h[3 + a];
+  }
+}
+
+double device_sum(osc_ctx* c, const double* v, int64_t n) {
+  DevBuf part(c->pool, sizeof(double) * (kRedBlocks + 1));
+  double* pp = part.as<double>();
+  const int nb = kRedBlocks;
+  hipLaunchKernelGGL(k_sum_partial, dim3(nb), dim3(kThreads), 0, c->stream, v, n, pp);
+  OSC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kThreads), 0, c->stream, pp, nb, pp + kRedBlocks);
+  OSC_HIP(hipGetLastError());
+  double r = 0.0;
+  OSC_HIP(hipMemcpyAsync(&r, pp + kRedBlocks, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  OSC_HIP(hipStreamSynchronize(c->stream));
+  return r;
+}
+
+static void exclusive_scan(osc_ctx* c, const uint32_t* in, uint32_t* out, int64_t n) {
+  const int64_t nb = (n + SC_TILE - 1) / SC_TILE;
+  DevBuf tot(c->pool, sizeof(uint32_t) * (nb + 1));
+  hipLaunchKernelGGL(k_scan_block, dim3(unsigned(nb)), dim3(SC_T), 0, c->stream, in, out, n,
+                     tot.as<uint32_t>());
+  OSC_HIP(hipGetLastError());
+  if (nb > 1) {
+    DevBuf tsc(c->pool, sizeof(uint32_t) * (nb + 1));
+    exclusive_scan(c, tot.as<const uint32_t>(), tsc.as<uint32_t>(), nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(unsigned(nb)), dim3(SC_T), 0, c->stream, out, n,
+                       tsc.as<const uint32_t>());
+    OSC_HIP(hipGetLastError());
+  }
+}
+
+void fill_u32(osc_ctx* c, uint32_t* p, int64_t n, uint32_t v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(nblk(n)), dim3(kThreads), 0, c->stream, p, n, v);
+  OSC_HIP(hipGetLastError());
+}
+
+// returns the device buffers holding the sorted (key, value) pairs
+static void radix_sort(osc_ctx* c, DevBuf& ka, DevBuf& va, DevBuf& kb, DevBuf& vb, int64_t n,
+                       int nbits, bool first_values_are_index, bool& result_in_b) {
+  const int npass = nbits <= 0 ? 0 : (nbits + 7) / 8;
+  const int nb = int((n + RS_TILE - 1) / RS_TILE);
+  result_in_b = false;
+  if (npass == 0 || n == 0) return;
+  DevBuf hist(c->pool, sizeof(uint32_t) * 256 * size_t(nb));
+  DevBuf offs(c->pool, sizeof(uint32_t) * 256 * size_t(nb));
+  uint32_t *kin = ka.as<uint32_t>(), *vin = va.as<uint32_t>();
+  uint32_t *kout = kb.as<uint32_t>(), *vout = vb.as<uint32_t>();
+  int shift = 0;
+  for (int p = 0; p < npass; ++p) {
+    const int bits = (nbits - shift + (npass - p) - 1) / (npass - p);
+    const uint32_t mask = (1u << bits) - 1u;
+    hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_T), 0, c->stream, kin, n, shift, mask,
+                       hist.as<uint32_t>(), nb);
+    OSC_HIP(hipGetLastError());
+    exclusive_scan(c, hist.as<const uint32_t>(), offs.as<uint32_t>(), int64_t(mask + 1) * nb);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_T), 0, c->stream, kin,
+                       (p == 0 && first_values_are_index) ? nullptr : vin, kout, vout, n, shift,
+                       bits, offs.as<const uint32_t>(), nb);
+    OSC_HIP(hipGetLastError());
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+    result_in_b = !result_in_b;
+    shift += bits;
+  }
+}
+
+int64_t deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho) {
+  const int nx = a.nn[0], ny = a.nn[1], nz = a.nn[2];
+  const int64_t ncell = int64_t(nx) * ny * nz;
+  OSC_REQUIRE(ncell < (int64_t(1) << 31) && a.n < (int64_t(1) << 32) - 1, OSC_ERR_ARG,
+              "grid or particle count too large for 32-bit keys");
+  uint32_t* cnt = static_cast<uint32_t*>(c->counters);  // [0] dropped, [1] heavy
+  fill_u32(c, cnt, 2, 0u);
+  const int64_t n = a.n;
+  DevBuf ka(c->pool, sizeof(uint32_t) * (n + 1)), va(c->pool, sizeof(uint32_t) * (n + 1));
+  DevBuf kb(c->pool, sizeof(uint32_t) * (n + 1)), vb(c->pool, sizeof(uint32_t) * (n + 1));
+  KeyParams kp{};
+  CellParams cp{};
+  for (int d = 0; d < 3; ++d) {
+    kp.mn[d] = a.mn[d];
+    kp.dinv[d] = 1.0 / a.delta[d];  // dxi = 1.d0/dx (:225-227)
+    kp.base[d] = a.harness ? double(a.nlo[d]) : 1.0;
+    kp.nlo[d] = a.nlo[d];
+    kp.nn[d] = a.nn[d];
+    cp.mn[d] = a.mn[d];
+    cp.d[d] = a.delta[d];
+    cp.dinv[d] = kp.dinv[d];
+    cp.wofs[d] = a.harness ? 1 : a.nlo[d];  // (ip-ilogbl+1)*dx  or  ip*dx
+    cp.nn[d] = a.nn[d];
+  }
+  kp.sentinel = uint32_t(ncell);
+  cp.hasq = a.q != nullptr;
+  cp.harness = a.harness;
+  cp.q_each = a.q_each;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_cic_keys, dim3(nblk(n)), dim3(kThreads), 0, c->stream, a.x, a.y, a.z, n,
+                       kp, ka.as<uint32_t>(), cnt);
+    OSC_HIP(hipGetLastError());
+  }
+  int nbits = 0;
+  while ((uint64_t(1) << nbits) <= uint64_t(ncell)) ++nbits;
+  bool in_b = false;
+  radix_sort(c, ka, va, kb, vb, n, nbits, true, in_b);
+  const uint32_t* sk = in_b ? kb.as<uint32_t>() : ka.as<uint32_t>();
+  const uint32_t* sv = in_b ? vb.as<uint32_t>() : va.as<uint32_t>();
+  DevBuf st(c->pool, sizeof(uint32_t) * ncell), en(c->pool, sizeof(uint32_t) * ncell);
+  fill_u32(c, st.as<uint32_t>(), ncell, 0u);
+  fill_u32(c, en.as<uint32_t>(), ncell, 0u);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk, n, cnt,
+                       st.as<uint32_t>(), en.as<uint32_t>());
+    OSC_HIP(hipGetLastError());
+  }
+  DevBuf S(c->pool, sizeof(double) * 8 * ncell), heavy(c->pool, sizeof(uint32_t) * ncell);
+  hipLaunchKernelGGL(k_cell_reduce, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream, sv,
+                     st.as<const uint32_t>(), en.as<const uint32_t>(), a.x, a.y, a.z, a.q, cp,
+                     S.as<double>(), heavy.as<uint32_t>(), cnt + 1);
+  OSC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_cell_heavy, dim3(2048), dim3(kThreads), 0, c->stream, sv,
+                     st.as<const uint32_t>(), en.as<const uint32_t>(), a.x, a.y, a.z, a.q, cp,
+                     S.as<double>(), heavy.as<const uint32_t>(), cnt + 1);
+  OSC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_node, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,
+                     S.as<const double>(), nx, ny, nz, a.harness, a.scale_after, rho);
+  OSC_HIP(hipGetLastError());
+  uint32_t nd = 0;
+  OSC_HIP(hipMemcpyAsync(&nd, cnt, sizeof(nd), hipMemcpyDeviceToHost, c->stream));
+  OSC_HIP(hipStreamSynchronize(c->stream));
+  return int64_t(nd);
+}
+
+}  // namespace osc

@@ -1,0 +1,132 @@
+// Definitions of the launchers for one precision; included by fft_f64.hip and
+// fft_f32.hip with OSC_REAL defined.
+#include "fft_launch.h"
+
+namespace osc {
+
+namespace {
+using T = OSC_REAL;
+
+inline dim3 grid3(int64_t x, int64_t y, int64_t z) {
+  return dim3((unsigned)x, (unsigned)y, (unsigned)z);
+}
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+size_t gen_lds_bytes(int L) { return size_t(LdsPad<T>::size(2 * kGenBX * L)) * sizeof(cpx<T>); }
+
+template <int L>
+void axis_t(int dir, const cpx<T>* in, cpx<T>* out, const AxisIO& a, int g1, int g2,
+            const cpx<T>* tw, hipStream_t s) {
+  constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
+  dim3 g = grid3(cdiv(a.nb, BX), g1, g2);
+  if (dir < 0)
+    hipLaunchKernelGGL((k_axis<T, L, -1>), g, dim3(NT), 0, s, in, out, a, tw);
+  else
+    hipLaunchKernelGGL((k_axis<T, L, 1>), g, dim3(NT), 0, s, in, out, a, tw);
+}
+template <int L>
+void xr2c_t(const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw, hipStream_t s) {
+  constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
+  dim3 g = grid3(cdiv(cdiv(a.nrows, 2), BX), 1, 1);
+  hipLaunchKernelGGL((k_xr2c<T, L>), g, dim3(NT), 0, s, in, out, a, tw);
+}
+template <int L>
+void xc2r_t(const cpx<T>* in, const C2RIO<T>& a, int nout, const cpx<T>* tw, hipStream_t s) {
+  constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
+  dim3 g = grid3(cdiv(cdiv(a.nrows, 2), BX), nout, 1);
+  hipLaunchKernelGGL((k_xc2r<T, L>), g, dim3(NT), 0, s, in, a, tw);
+}
+template <int L>
+void zinv_t(const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, int Ly, int nout, const cpx<T>* tw,
+            hipStream_t s) {
+  constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
+  dim3 g = grid3(cdiv(a.nb, BX), Ly, nout);
+  hipLaunchKernelGGL((k_zinv<T, L>), g, dim3(NT), 0, s, rho, out, a, tw);
+}
+}  // namespace
+
+template <>
+void launch_axis<T>(int L, int dir, const cpx<T>* in, cpx<T>* out, const AxisIO& a, int g1, int g2,
+                    const cpx<T>* tw, hipStream_t s) {
+  switch (L) {
+#define OSC_CASE(LL)                           \
+  case LL:                                     \
+    axis_t<LL>(dir, in, out, a, g1, g2, tw, s); \
+    break;
+    OSC_FOR_EACH_L(OSC_CASE)
+#undef OSC_CASE
+    default: {
+      GenericPlan pl = generic_plan(L);
+      dim3 g = grid3(cdiv(a.nb, kGenBX), g1, g2);
+      if (dir < 0)
+        hipLaunchKernelGGL((k_axis_gen<T, -1>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in,
+                           out, a, tw, pl);
+      else
+        hipLaunchKernelGGL((k_axis_gen<T, 1>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in,
+                           out, a, tw, pl);
+    }
+  }
+  OSC_HIP(hipGetLastError());
+}
+
+template <>
+void launch_xr2c<T>(int L, const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw,
+                    hipStream_t s) {
+  switch (L) {
+#define OSC_CASE(LL)                    \
+  case LL:                              \
+    xr2c_t<LL>(in, out, a, tw, s);      \
+    break;
+    OSC_FOR_EACH_L(OSC_CASE)
+#undef OSC_CASE
+    default: {
+      GenericPlan pl = generic_plan(L);
+      dim3 g = grid3(cdiv(cdiv(a.nrows, 2), kGenBX), 1, 1);
+      hipLaunchKernelGGL((k_xr2c_gen<T>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in, out,
+                         a, tw, pl);
+    }
+  }
+  OSC_HIP(hipGetLastError());
+}
+
+template <>
+void launch_xc2r<T>(int L, const cpx<T>* in, const C2RIO<T>& a, int nout, const cpx<T>* tw,
+                    hipStream_t s) {
+  switch (L) {
+#define OSC_CASE(LL)                  \
+  case LL:                            \
+    xc2r_t<LL>(in, a, nout, tw, s);   \
+    break;
+    OSC_FOR_EACH_L(OSC_CASE)
+#undef OSC_CASE
+    default: {
+      GenericPlan pl = generic_plan(L);
+      dim3 g = grid3(cdiv(cdiv(a.nrows, 2), kGenBX), nout, 1);
+      hipLaunchKernelGGL((k_xc2r_gen<T>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in, a,
+                         tw, pl);
+    }
+  }
+  OSC_HIP(hipGetLastError());
+}
+
+template <>
+void launch_zinv<T>(int L, const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, int Ly, int nout,
+                    const cpx<T>* tw, hipStream_t s) {
+  switch (L) {
+#define OSC_CASE(LL)                              \
+  case LL:                                        \
+    zinv_t<LL>(rho, out, a, Ly, nout, tw, s);     \
+    break;
+    OSC_FOR_EACH_L(OSC_CASE)
+#undef OSC_CASE
+    default: {
+      GenericPlan pl = generic_plan(L);
+      dim3 g = grid3(cdiv(a.nb, kGenBX), Ly, nout);
+      hipLaunchKernelGGL((k_zinv_gen<T>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, rho, out,
+                         a, tw, pl);
+    }
+  }
+  OSC_HIP(hipGetLastError());
+}
+
+}  // namespace osc

@@ -1,0 +1,368 @@
+// Pass kernels of the doubled-grid convolution pipeline and the generic
+// per-axis complex FFT, built on fft_core.h.  Instantiated per precision in
+// fft_f64.hip / fft_f32.hip.
+//
+// Spectrum layout: half spectrum S[kz][ky][kx], kx fastest with a padded row
+// of KXP >= L/2+1 entries, ky rows of Ly, kz planes of Lz.
+#pragma once
+#include "fft_core.h"
+
+namespace osc {
+
+// ---------------------------------------------------------------------------
+// argument blocks
+// ---------------------------------------------------------------------------
+// Batched strided line FFT.  Element (line b, position p) of grid block
+// (o1, o2) is at base + b*s_b + p*s_p + o1*s_o1 + o2*s_o2 (input) and
+// b*d_b + (p-w0)*d_p + o1*d_o1 + o2*d_o2 (output).  Positions >= n_in are
+// zero and not read; only positions in [w0, w0+n_out) are written.
+struct AxisIO {
+  int64_t s_b, s_p, s_o1, s_o2;
+  int64_t d_b, d_p, d_o1, d_o2;
+  int nb, n_in, w0, n_out;
+};
+
+// Real rows -> half spectra.  Row r (0 <= r < nrows) is (y, z) = (r % ny_rows,
+// r / ny_rows); input row at in + y*in_sy + z*in_sz (n_in reals, rest zero),
+// output row at out + y*out_sy + z*out_sz (L/2+1 complex).  Two rows are
+// transformed together as the real and imaginary parts of one complex line.
+struct R2CIO {
+  int64_t in_sy, in_sz, out_sy, out_sz;
+  int64_t nrows;
+  int ny_rows, n_in;
+};
+
+// Epilogue of the inverse x pass for one output field: e = s1*val is stored
+// to dst1 (if any); dst2 (if any) receives bmode 0: s2*e, or
+// bmode 1: ((sgn*e)*beta)/c   (the two B-field formulas of the upstream code).
+struct Epilogue {
+  double* dst1;
+  double* dst2;
+  double s1, s2, beta, c;
+  int bmode, pad_;
+};
+
+constexpr int kMaxOut = 6;
+constexpr int kMaxTerm = 4;
+
+// Half spectra -> real rows.  Rows as in R2CIO (spectrum side is "in"),
+// output position x in [w0, w0+n_out) stored at dst + x-w0 + row offsets.
+template <typename T>
+struct C2RIO {
+  int64_t in_sy, in_sz, in_so;
+  int64_t out_sy, out_sz;
+  int64_t nrows;
+  int ny_rows, w0, n_out, pad_;
+  Epilogue ep[kMaxOut];
+};
+
+// One spectral term: sgn * rho_hat(mirror(k)) * G(k)
+template <typename T>
+struct Term {
+  const cpx<T>* g;
+  T sgn;
+  int mir;  // bit 0: x, bit 1: y, bit 2: z
+};
+template <typename T>
+struct OutTerms {
+  Term<T> t[kMaxTerm];
+  int nt;
+};
+template <typename T>
+struct ZinvIO {
+  int64_t sy, sz;         // spectrum strides (KXP, KXP*Ly)
+  int64_t o_sy, o_sz, o_so;  // output strides
+  int Ly, Lz, nb, w0, n_out, pad_;
+  OutTerms<T> out[kMaxOut];
+};
+
+// ---------------------------------------------------------------------------
+// launch geometry of the compile-time engine
+// ---------------------------------------------------------------------------
+template <typename T, int L>
+struct Geo {
+  static constexpr int TPL = PlanInfo<L>::TPL;
+  static constexpr int bx_of(int tpl) {
+    int bx = 256 / tpl;
+    if (bx < 1) bx = 1;
+    while (bx > 1 && bx * L * (int)sizeof(cpx<T>) > 49152) bx /= 2;
+    return bx;
+  }
+  static constexpr int BX = PlanInfo<L>::ns ? bx_of(TPL) : 1;
+  static constexpr int NT = BX * TPL;
+};
+
+// generic engine geometry: one line per 64-thread workgroup
+constexpr int kGenTPL = 64;
+constexpr int kGenBX = 1;
+
+template <typename T>
+__device__ __forceinline__ cpx<T> ldc(const cpx<T>* p) {
+  return *p;
+}
+
+// ---------------------------------------------------------------------------
+// strided line FFT (y or z axis, or x with s_p = 1)
+// ---------------------------------------------------------------------------
+template <typename T, int L, int DIR>
+__global__ void __launch_bounds__((Geo<T, L>::NT))
+    k_axis(const cpx<T>* __restrict__ in, cpx<T>* out, AxisIO a, const cpx<T>* __restrict__ tw) {
+  constexpr int BX = Geo<T, L>::BX;
+  using F = FftTile<T, L, DIR, BX, false>;
+  __shared__ cpx<T> lds[F::LDS];
+  F f;
+  const int bg = blockIdx.x * BX + f.b;
+  const bool ok = bg < a.nb;
+  const cpx<T>* ib = in + (ok ? bg * a.s_b + blockIdx.y * a.s_o1 + blockIdx.z * a.s_o2 : 0);
+  f.load([&](int, int p) { return (ok && p < a.n_in) ? ib[p * a.s_p] : mk<T>(0, 0); });
+  f.run(lds, tw);
+  cpx<T>* ob = out + (ok ? bg * a.d_b + blockIdx.y * a.d_o1 + blockIdx.z * a.d_o2 : 0);
+  f.store([&](int, int p, cpx<T> v) {
+    const int q = p - a.w0;
+    if (ok && q >= 0 && q < a.n_out) ob[q * a.d_p] = v;
+  });
+}
+
+template <typename T, int DIR>
+__global__ void __launch_bounds__(kGenBX* kGenTPL)
+    k_axis_gen(const cpx<T>* __restrict__ in, cpx<T>* out, AxisIO a, const cpx<T>* __restrict__ tw,
+               GenericPlan pl) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
+  FftGeneric<T, DIR, kGenBX, kGenTPL> f(pl.L);
+  const int bg = blockIdx.x * kGenBX + f.b;
+  const bool ok = bg < a.nb;
+  const cpx<T>* ib = in + (ok ? bg * a.s_b + blockIdx.y * a.s_o1 + blockIdx.z * a.s_o2 : 0);
+  f.load(lds, [&](int, int p) { return (ok && p < a.n_in) ? ib[p * a.s_p] : mk<T>(0, 0); });
+  f.run(pl, tw);
+  cpx<T>* ob = out + (ok ? bg * a.d_b + blockIdx.y * a.d_o1 + blockIdx.z * a.d_o2 : 0);
+  f.store([&](int, int p, cpx<T> v) {
+    const int q = p - a.w0;
+    if (ok && q >= 0 && q < a.n_out) ob[q * a.d_p] = v;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// forward x pass: pairs of real rows -> half spectra
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void r2c_unpack(cpx<T> zk, cpx<T> zm, cpx<T>& A, cpx<T>& B) {
+  // A = (Z[k] + conj Z[L-k]) / 2 ;  B = (Z[k] - conj Z[L-k]) / (2i)
+  cpx<T> s = add(zk, conjg(zm)), d = sub(zk, conjg(zm));
+  A = scale(s, T(0.5));
+  B = mk(T(0.5) * d.y, T(-0.5) * d.x);
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__((Geo<T, L>::NT))
+    k_xr2c(const double* __restrict__ in, cpx<T>* __restrict__ out, R2CIO a,
+           const cpx<T>* __restrict__ tw) {
+  constexpr int BX = Geo<T, L>::BX;
+  using F = FftTile<T, L, -1, BX, true>;
+  __shared__ cpx<T> lds[F::LDS];
+  F f;
+  const int64_t ra = 2 * (int64_t(blockIdx.x) * BX + f.b), rb = ra + 1;
+  const bool oka = ra < a.nrows, okb = rb < a.nrows;
+  const double* pa = in + (oka ? (ra % a.ny_rows) * a.in_sy + (ra / a.ny_rows) * a.in_sz : 0);
+  const double* pb = in + (okb ? (rb % a.ny_rows) * a.in_sy + (rb / a.ny_rows) * a.in_sz : 0);
+  f.load([&](int, int p) {
+    T re = (oka && p < a.n_in) ? T(pa[p]) : T(0);
+    T im = (okb && p < a.n_in) ? T(pb[p]) : T(0);
+    return mk(re, im);
+  });
+  f.run(lds, tw);
+  __syncthreads();
+  f.dump(lds);
+  __syncthreads();
+  cpx<T>* oa = out + (oka ? (ra % a.ny_rows) * a.out_sy + (ra / a.ny_rows) * a.out_sz : 0);
+  cpx<T>* ob = out + (okb ? (rb % a.ny_rows) * a.out_sy + (rb / a.ny_rows) * a.out_sz : 0);
+  for (int k = f.t; k <= L / 2; k += F::TPL) {
+    cpx<T> A, B;
+    r2c_unpack(lds[F::lidx(f.b, k)], lds[F::lidx(f.b, (L - k) % L)], A, B);
+    if (oka) oa[k] = A;
+    if (okb) ob[k] = B;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kGenBX* kGenTPL)
+    k_xr2c_gen(const double* __restrict__ in, cpx<T>* __restrict__ out, R2CIO a,
+               const cpx<T>* __restrict__ tw, GenericPlan pl) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
+  using F = FftGeneric<T, -1, kGenBX, kGenTPL>;
+  F f(pl.L);
+  const int L = pl.L;
+  const int64_t ra = 2 * (int64_t(blockIdx.x) * kGenBX + f.b), rb = ra + 1;
+  const bool oka = ra < a.nrows, okb = rb < a.nrows;
+  const double* pa = in + (oka ? (ra % a.ny_rows) * a.in_sy + (ra / a.ny_rows) * a.in_sz : 0);
+  const double* pb = in + (okb ? (rb % a.ny_rows) * a.in_sy + (rb / a.ny_rows) * a.in_sz : 0);
+  f.load(lds, [&](int, int p) {
+    T re = (oka && p < a.n_in) ? T(pa[p]) : T(0);
+    T im = (okb && p < a.n_in) ? T(pb[p]) : T(0);
+    return mk(re, im);
+  });
+  f.run(pl, tw);
+  const int buf = f.result_buf();
+  cpx<T>* oa = out + (oka ? (ra % a.ny_rows) * a.out_sy + (ra / a.ny_rows) * a.out_sz : 0);
+  cpx<T>* ob = out + (okb ? (rb % a.ny_rows) * a.out_sy + (rb / a.ny_rows) * a.out_sz : 0);
+  for (int k = f.t; k <= L / 2; k += kGenTPL) {
+    cpx<T> A, B;
+    r2c_unpack(lds[f.lidx(f.b, k, buf)], lds[f.lidx(f.b, (L - k) % L, buf)], A, B);
+    if (oka) oa[k] = A;
+    if (okb) ob[k] = B;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// inverse x pass: pairs of half spectra -> real rows, with FP64 epilogue
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ cpx<T> c2r_pack(const cpx<T>* ra, const cpx<T>* rb, bool okb, int p,
+                                           int L) {
+  // Z[p] = A[p] + i B[p] with the Hermitian extension A[L-k] = conj A[k];
+  // the imaginary parts of the self-conjugate bins (0 and L/2) are dropped.
+  const bool hi = 2 * p > L;
+  const int k = hi ? L - p : p;
+  cpx<T> A = ra[k];
+  cpx<T> B = okb ? rb[k] : mk<T>(0, 0);
+  if (k == 0 || 2 * k == L) {
+    A.y = 0;
+    B.y = 0;
+  }
+  if (hi) {
+    A.y = -A.y;
+    B.y = -B.y;
+  }
+  return mk(A.x - B.y, A.y + B.x);
+}
+
+__device__ __forceinline__ void epi_store(const Epilogue& e, int64_t off, double val) {
+  const double v = e.s1 * val;
+  if (e.dst1) e.dst1[off] = v;
+  if (e.dst2) e.dst2[off] = e.bmode == 0 ? e.s2 * v : ((e.s2 * v) * e.beta) / e.c;
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__((Geo<T, L>::NT))
+    k_xc2r(const cpx<T>* __restrict__ in, C2RIO<T> a, const cpx<T>* __restrict__ tw) {
+  constexpr int BX = Geo<T, L>::BX;
+  using F = FftTile<T, L, 1, BX, true>;
+  __shared__ cpx<T> lds[F::LDS];
+  F f;
+  const int o = blockIdx.y;
+  const int64_t ra = 2 * (int64_t(blockIdx.x) * BX + f.b), rb = ra + 1;
+  const bool oka = ra < a.nrows, okb = rb < a.nrows;
+  const cpx<T>* base = in + o * a.in_so;
+  const cpx<T>* pa = base + (oka ? (ra % a.ny_rows) * a.in_sy + (ra / a.ny_rows) * a.in_sz : 0);
+  const cpx<T>* pb = base + (okb ? (rb % a.ny_rows) * a.in_sy + (rb / a.ny_rows) * a.in_sz : 0);
+  f.load([&](int, int p) { return oka ? c2r_pack(pa, pb, okb, p, L) : mk<T>(0, 0); });
+  f.run(lds, tw);
+  const Epilogue e = a.ep[o];
+  const int64_t offa = (ra % a.ny_rows) * a.out_sy + (ra / a.ny_rows) * a.out_sz;
+  const int64_t offb = (rb % a.ny_rows) * a.out_sy + (rb / a.ny_rows) * a.out_sz;
+  f.store([&](int, int p, cpx<T> v) {
+    const int x = p - a.w0;
+    if (x < 0 || x >= a.n_out) return;
+    if (oka) epi_store(e, offa + x, double(v.x));
+    if (okb) epi_store(e, offb + x, double(v.y));
+  });
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kGenBX* kGenTPL)
+    k_xc2r_gen(const cpx<T>* __restrict__ in, C2RIO<T> a, const cpx<T>* __restrict__ tw,
+               GenericPlan pl) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
+  FftGeneric<T, 1, kGenBX, kGenTPL> f(pl.L);
+  const int L = pl.L;
+  const int o = blockIdx.y;
+  const int64_t ra = 2 * (int64_t(blockIdx.x) * kGenBX + f.b), rb = ra + 1;
+  const bool oka = ra < a.nrows, okb = rb < a.nrows;
+  const cpx<T>* base = in + o * a.in_so;
+  const cpx<T>* pa = base + (oka ? (ra % a.ny_rows) * a.in_sy + (ra / a.ny_rows) * a.in_sz : 0);
+  const cpx<T>* pb = base + (okb ? (rb % a.ny_rows) * a.in_sy + (rb / a.ny_rows) * a.in_sz : 0);
+  f.load(lds, [&](int, int p) { return oka ? c2r_pack(pa, pb, okb, p, L) : mk<T>(0, 0); });
+  f.run(pl, tw);
+  const Epilogue e = a.ep[o];
+  const int64_t offa = (ra % a.ny_rows) * a.out_sy + (ra / a.ny_rows) * a.out_sz;
+  const int64_t offb = (rb % a.ny_rows) * a.out_sy + (rb / a.ny_rows) * a.out_sz;
+  f.store([&](int, int p, cpx<T> v) {
+    const int x = p - a.w0;
+    if (x < 0 || x >= a.n_out) return;
+    if (oka) epi_store(e, offa + x, double(v.x));
+    if (okb) epi_store(e, offb + x, double(v.y));
+  });
+}
+
+// ---------------------------------------------------------------------------
+// spectral combine + inverse z pass
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ cpx<T> combine(const cpx<T>* __restrict__ rho, const ZinvIO<T>& a,
+                                          const OutTerms<T>& ot, int kx, int ky, int kz) {
+  cpx<T> acc = mk<T>(0, 0);
+  const int64_t gi = kx + ky * a.sy + kz * a.sz;
+  for (int t = 0; t < ot.nt; ++t) {
+    const Term<T> tm = ot.t[t];
+    bool my, mz, cj;
+    if (tm.mir & 1) {  // rho_hat(-kx,ky',kz') = conj(rho_hat(kx,-ky',-kz'))
+      my = !(tm.mir & 2);
+      mz = !(tm.mir & 4);
+      cj = true;
+    } else {
+      my = tm.mir & 2;
+      mz = tm.mir & 4;
+      cj = false;
+    }
+    const int yy = my ? (ky == 0 ? 0 : a.Ly - ky) : ky;
+    const int zz = mz ? (kz == 0 ? 0 : a.Lz - kz) : kz;
+    cpx<T> r = rho[kx + yy * a.sy + zz * a.sz];
+    if (cj) r.y = -r.y;
+    cpx<T> p = mul(r, tm.g[gi]);
+    acc = add(acc, mk(tm.sgn * p.x, tm.sgn * p.y));
+  }
+  return acc;
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__((Geo<T, L>::NT))
+    k_zinv(const cpx<T>* __restrict__ rho, cpx<T>* __restrict__ out, ZinvIO<T> a,
+           const cpx<T>* __restrict__ tw) {
+  constexpr int BX = Geo<T, L>::BX;
+  using F = FftTile<T, L, 1, BX, false>;
+  __shared__ cpx<T> lds[F::LDS];
+  F f;
+  const int kx = blockIdx.x * BX + f.b, ky = blockIdx.y, o = blockIdx.z;
+  const bool ok = kx < a.nb;
+  const OutTerms<T>& ot = a.out[o];
+  f.load([&](int, int p) { return ok ? combine(rho, a, ot, kx, ky, p) : mk<T>(0, 0); });
+  f.run(lds, tw);
+  cpx<T>* ob = out + o * a.o_so + kx + ky * a.o_sy;
+  f.store([&](int, int p, cpx<T> v) {
+    const int q = p - a.w0;
+    if (ok && q >= 0 && q < a.n_out) ob[q * a.o_sz] = v;
+  });
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kGenBX* kGenTPL)
+    k_zinv_gen(const cpx<T>* __restrict__ rho, cpx<T>* __restrict__ out, ZinvIO<T> a,
+               const cpx<T>* __restrict__ tw, GenericPlan pl) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
+  FftGeneric<T, 1, kGenBX, kGenTPL> f(pl.L);
+  const int kx = blockIdx.x * kGenBX + f.b, ky = blockIdx.y, o = blockIdx.z;
+  const bool ok = kx < a.nb;
+  const OutTerms<T>& ot = a.out[o];
+  f.load(lds, [&](int, int p) { return ok ? combine(rho, a, ot, kx, ky, p) : mk<T>(0, 0); });
+  f.run(pl, tw);
+  cpx<T>* ob = out + o * a.o_so + kx + ky * a.o_sy;
+  f.store([&](int, int p, cpx<T> v) {
+    const int q = p - a.w0;
+    if (ok && q >= 0 && q < a.n_out) ob[q * a.o_sz] = v;
+  });
+}
+
+}  // namespace osc

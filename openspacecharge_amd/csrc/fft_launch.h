@@ -1,0 +1,31 @@
+// Host-side launchers of the FFT pass kernels (defined per precision in
+// fft_f64.hip / fft_f32.hip).
+#pragma once
+#include "fft_kernels.h"
+
+namespace osc {
+
+// lengths with a compile-time plan
+#define OSC_FOR_EACH_L(X)                                                                   \
+  X(16) X(24) X(32) X(36) X(48) X(54) X(64) X(72) X(96) X(108) X(128) X(144) X(162) X(192) \
+      X(216) X(256) X(288) X(324) X(384) X(432) X(512) X(576) X(648) X(768) X(864) X(1024)
+
+bool fft_has_plan(int L);
+// true when every prime factor of L is <= 23 and the generic engine fits LDS
+bool fft_len_supported(int L);
+GenericPlan generic_plan(int L);
+
+template <typename T>
+void launch_axis(int L, int dir, const cpx<T>* in, cpx<T>* out, const AxisIO& a, int g1, int g2,
+                 const cpx<T>* tw, hipStream_t s);
+template <typename T>
+void launch_xr2c(int L, const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw,
+                 hipStream_t s);
+template <typename T>
+void launch_xc2r(int L, const cpx<T>* in, const C2RIO<T>& a, int nout, const cpx<T>* tw,
+                 hipStream_t s);
+template <typename T>
+void launch_zinv(int L, const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, int Ly, int nout,
+                 const cpx<T>* tw, hipStream_t s);
+
+}  // namespace osc

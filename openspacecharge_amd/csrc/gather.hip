@@ -1,0 +1,109 @@
+// Trilinear field gather: interpolate_field (open_spacecharge_mod.f90:311-390)
+// for a batch of points, bit-compatible with the upstream per-point routine.
+#include "osc_internal.h"
+
+namespace osc {
+
+namespace {
+constexpr int kThreads = 256;
+
+struct IP {
+  double mn[3], delta[3], hinv[3];
+  int nlo[3], nhi[3], nn[3];
+};
+
+__device__ __forceinline__ int clamp_idx(double x, double mn, double hinv, int nhi, bool& fl) {
+  // ip = floor((x-min)*hxi+1); clamp to [1, nhi-1] (:320-355)
+  const double f = floor((x - mn) * hinv + 1.0);
+  int ip;
+  if (!(f >= 1.0)) {
+    ip = 1;
+    fl = true;
+  } else if (f > double(nhi - 1)) {
+    ip = nhi - 1;
+    fl = true;
+  } else {
+    ip = int(f);
+  }
+  return ip;
+}
+
+__device__ __forceinline__ double interp8(const double* __restrict__ F, int64_t sy, int64_t sz,
+                                          int64_t b, double ab, double de, double gh) {
+  // E = sum of 8 terms in the upstream order (:369-376)
+  const int64_t i1 = 1, j1 = sy, k1 = sz;
+  return F[b] * ab * de * gh + F[b + j1] * ab * (1. - de) * gh +
+         F[b + j1 + k1] * ab * (1. - de) * (1. - gh) + F[b + k1] * ab * de * (1. - gh) +
+         F[b + i1 + k1] * (1. - ab) * de * (1. - gh) +
+         F[b + i1 + j1 + k1] * (1. - ab) * (1. - de) * (1. - gh) +
+         F[b + i1 + j1] * (1. - ab) * (1. - de) * gh + F[b + i1] * (1. - ab) * de * gh;
+}
+
+__global__ void k_interp(const double* __restrict__ ef, const double* __restrict__ bf, int64_t n,
+                         const double* __restrict__ x, const double* __restrict__ y,
+                         const double* __restrict__ z, IP p, double* __restrict__ E,
+                         double* __restrict__ B, int8_t* __restrict__ clamped,
+                         uint32_t* __restrict__ nclamp) {
+  __shared__ uint32_t cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  uint32_t mine = 0;
+  const int64_t sy = p.nn[0], sz = int64_t(p.nn[0]) * p.nn[1], nt = sz * p.nn[2];
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double xi = x[i], yi = y[i], zi = z[i];
+    bool fl = false;
+    const int ip = clamp_idx(xi, p.mn[0], p.hinv[0], p.nhi[0], fl);
+    const int jp = clamp_idx(yi, p.mn[1], p.hinv[1], p.nhi[1], fl);
+    const int kp = clamp_idx(zi, p.mn[2], p.hinv[2], p.nhi[2], fl);
+    const double ab = ((p.mn[0] - xi) + ip * p.delta[0]) * p.hinv[0];
+    const double de = ((p.mn[1] - yi) + jp * p.delta[1]) * p.hinv[1];
+    const double gh = ((p.mn[2] - zi) + kp * p.delta[2]) * p.hinv[2];
+    // array element (ip, jp, kp) of a field with lower bound nlo; kept inside
+    // the array when nlo differs from 1 (upstream assumes nlo = 1)
+    int i0 = ip - p.nlo[0], j0 = jp - p.nlo[1], k0 = kp - p.nlo[2];
+    i0 = i0 < 0 ? 0 : (i0 > p.nn[0] - 2 ? p.nn[0] - 2 : i0);
+    j0 = j0 < 0 ? 0 : (j0 > p.nn[1] - 2 ? p.nn[1] - 2 : j0);
+    k0 = k0 < 0 ? 0 : (k0 > p.nn[2] - 2 ? p.nn[2] - 2 : k0);
+    const int64_t b = i0 + j0 * sy + k0 * sz;
+    if (E && ef)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) E[3 * i + c] = interp8(ef + c * nt, sy, sz, b, ab, de, gh);
+    if (B && bf)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) B[3 * i + c] = interp8(bf + c * nt, sy, sz, b, ab, de, gh);
+    if (clamped) clamped[i] = fl ? 1 : 0;
+    mine += fl ? 1u : 0u;
+  }
+  if (mine) atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
+}
+}  // namespace
+
+int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
+                    int64_t n, const double* x, const double* y, const double* z, double* E,
+                    double* B, int8_t* clamped) {
+  IP p{};
+  for (int d = 0; d < 3; ++d) {
+    p.mn[d] = a.mn[d];
+    p.delta[d] = a.delta[d];
+    p.hinv[d] = 1.0 / a.delta[d];  // hxi = 1.d0/delta (:319)
+    p.nlo[d] = a.nlo[d];
+    p.nhi[d] = a.nhi[d];
+    p.nn[d] = a.nn[d];
+    OSC_REQUIRE(a.nn[d] >= 2, OSC_ERR_ARG, "interpolation needs at least 2 nodes per axis");
+  }
+  uint32_t* cnt = static_cast<uint32_t*>(c->counters) + 2;
+  fill_u32(c, cnt, 1, 0u);
+  if (n > 0) {
+    int64_t nb = (n + kThreads - 1) / kThreads;
+    if (nb > 65536 * 8) nb = 65536 * 8;
+    hipLaunchKernelGGL(k_interp, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream, ef, bf, n, x,
+                       y, z, p, E, B, clamped, cnt);
+    OSC_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+}  // namespace osc

@@ -1,0 +1,309 @@
+// Green-function builders (FP64, on the device), their R2C spectra, the
+// spectrum cache, and conversions to the upstream full-spectrum convention.
+#include "igf_math.h"
+#include "osc_internal.h"
+
+namespace osc {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline int64_t nblk(int64_t n, int t = kThreads) {
+  int64_t b = (n + t - 1) / t;
+  return b > 65536 * 8 ? 65536 * 8 : b;
+}
+
+// primitive of osc_get_cgrn_freespace on the offset grid
+// (open_spacecharge_mod.f90:674-691): F = prim(u,v,w) * factor
+__global__ void k_prim_offset(double* __restrict__ F, int L0, int L1, int L2, double dx, double dy,
+                              double dz, double umin, double vmin, double wmin, double factor,
+                              int icomp) {
+  const int64_t N = int64_t(L0) * L1 * L2;
+  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < N;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    const int i = int(idx % L0);
+    const int j = int((idx / L0) % L1);
+    const int k = int(idx / (int64_t(L0) * L1));
+    const double w = double(k) * dz + wmin;
+    const double v = double(j) * dy + vmin;
+    const double u = double(i) * dx + umin;
+    double g;
+    if (icomp == 0) g = lafun2(u, v, w) * factor;
+    else if (icomp == 1) g = xlafun2(u, v, w) * factor;
+    else if (icomp == 2) g = xlafun2(v, w, u) * factor;
+    else g = xlafun2(w, u, v) * factor;
+    F[idx] = g;
+  }
+}
+
+// 8-corner difference over 1:size-1 with the last planes left raw
+// (open_spacecharge_mod.f90:697-705); output z index circularly shifted by sh
+__global__ void k_diff_offset(const double* __restrict__ F, double* __restrict__ G, int L0, int L1,
+                              int L2, int sh) {
+  const int64_t N = int64_t(L0) * L1 * L2;
+  const int64_t P = int64_t(L0) * L1;
+  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < N;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    const int i = int(idx % L0);
+    const int j = int((idx / L0) % L1);
+    const int kq = int(idx / P);
+    int k = (kq - sh) % L2;
+    if (k < 0) k += L2;
+    const int64_t b = i + int64_t(L0) * j + P * k;
+    double g;
+    if (i < L0 - 1 && j < L1 - 1 && k < L2 - 1) {
+      g = F[b + 1 + L0 + P] - F[b + L0 + P] - F[b + 1 + P] - F[b + 1 + L0] - F[b] + F[b + P] +
+          F[b + L0] + F[b + 1];
+    } else {
+      g = F[b];
+    }
+    G[idx] = g;
+  }
+}
+
+// legacy loops: osc_getgrnfree (core:476-493), osc_getgrnimageshift
+// (core:1157-1174), osc_getgrnimageconvcorr (core:1207-1226)
+__global__ void k_green_legacy(double* __restrict__ G, GreenSpec s) {
+  const int L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
+  const int64_t N = int64_t(L0) * L1 * L2;
+  const double dx = s.delta[0], dy = s.delta[1], dz = s.delta[2];
+  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < N;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    const int ii = int(idx % L0);
+    const int jj = int((idx / L0) % L1);
+    const int kq = int(idx / (int64_t(L0) * L1));
+    int kk = (kq - s.out_shift_z) % L2;
+    if (kk < 0) kk += L2;
+    double w;
+    if (s.kind == GK_IMG_CONVCORR) {
+      const int kp = s.lo[2] + kk;
+      w = kp * dz + 2.0 * s.umin[2] - 2.0 * s.nlo3 * s.delta[2];
+    } else {
+      const int kp = s.lo[2] + (kk + s.shift[2]) % L2;
+      w = kp * dz;
+      if (s.kind == GK_IMG_SHIFT) w = w + s.zshft;
+    }
+    const int jp = s.lo[1] + (jj + s.shift[1]) % L1;
+    const double v = jp * dy;
+    const int ip = s.lo[0] + (ii + s.shift[0]) % L0;
+    const double u = ip * dx;
+    G[idx] = igf_legacy(s.igf, s.icomp, u, v, w, s.gamma, dx, dy, dz);
+  }
+}
+
+// osc_getgrnpipe loops (core:652-713), term t = 0..3
+__global__ void k_green_pipe(double* __restrict__ G, GreenSpec s) {
+  const int L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
+  const int64_t N = int64_t(L0) * L1 * L2;
+  const double hx = s.delta[0], hy = s.delta[1], hz = s.delta[2];
+  const int si = (s.term & 2) ? -1 : 1;
+  const int sj = (s.term & 1) ? -1 : 1;
+  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < N;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    const int ii = int(idx % L0);
+    const int jj = int((idx / L0) % L1);
+    const int kk = int(idx / (int64_t(L0) * L1));
+    const int kp = s.lo[2] + (kk + s.shift[2]) % L2;
+    const double w = kp * hz;
+    double v, u;
+    if (sj == 1) {
+      const int jp = s.lo[1] + (jj + s.shift[1]) % L1;
+      v = jp * hy;
+    } else {
+      v = 2.0 * s.umin[1] + jj * hy;
+    }
+    if (si == 1) {
+      const int ip = s.lo[0] + (ii + s.shift[0]) % L0;
+      u = ip * hx;
+    } else {
+      u = 2.0 * s.umin[0] + ii * hx;
+    }
+    G[idx] = rfun(u, v, w, s.gamma, s.a, s.b, hz, si, sj);
+  }
+}
+
+__global__ void k_convert(const cpx<double>* __restrict__ in, cpx<float>* __restrict__ out,
+                          int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    out[i] = mk(float(in[i].x), float(in[i].y));
+}
+
+}  // namespace
+
+void build_green_real(osc_ctx* c, const GreenSpec& s, double* g) {
+  const int64_t N = int64_t(s.L[0]) * s.L[1] * s.L[2];
+  if (s.kind == GK_OFFSET) {
+    // osc_get_cgrn_freespace (open_spacecharge_mod.f90:633-707)
+    const double dx = s.delta[0], dy = s.delta[1], dz = s.delta[2] * s.gamma;
+    const double factor =
+        (s.icomp == 1 || s.icomp == 2) ? s.gamma / (dx * dy * dz) : 1.0 / (dx * dy * dz);
+    double umin = (0.5 - double(s.L[0] / 2)) * dx;
+    double vmin = (0.5 - double(s.L[1] / 2)) * dy;
+    double wmin = (0.5 - double(s.L[2] / 2)) * dz;
+    umin = umin + s.offset[0];
+    vmin = vmin + s.offset[1];
+    wmin = wmin + s.offset[2] * s.gamma;
+    DevBuf F(c->pool, sizeof(double) * N);
+    hipLaunchKernelGGL(k_prim_offset, dim3(nblk(N)), dim3(kThreads), 0, c->stream, F.as<double>(),
+                       s.L[0], s.L[1], s.L[2], dx, dy, dz, umin, vmin, wmin, factor, s.icomp);
+    OSC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_diff_offset, dim3(nblk(N)), dim3(kThreads), 0, c->stream,
+                       F.as<const double>(), g, s.L[0], s.L[1], s.L[2], s.out_shift_z);
+    OSC_HIP(hipGetLastError());
+  } else if (s.kind == GK_PIPE) {
+    hipLaunchKernelGGL(k_green_pipe, dim3(nblk(N)), dim3(kThreads), 0, c->stream, g, s);
+    OSC_HIP(hipGetLastError());
+  } else {
+    hipLaunchKernelGGL(k_green_legacy, dim3(nblk(N)), dim3(kThreads), 0, c->stream, g, s);
+    OSC_HIP(hipGetLastError());
+  }
+}
+
+void spectrum_forward_full(osc_ctx* c, const double* g, cpx<double>* spec, const SpecDims& d) {
+  const int L0 = d.L[0], L1 = d.L[1], L2 = d.L[2];
+  R2CIO r{};
+  r.in_sy = L0;
+  r.in_sz = int64_t(L0) * L1;
+  r.out_sy = d.KXP;
+  r.out_sz = int64_t(d.KXP) * L1;
+  r.nrows = int64_t(L1) * L2;
+  r.ny_rows = L1;
+  r.n_in = L0;
+  launch_xr2c<double>(L0, g, spec, r, twiddles<double>(c, L0), c->stream);
+  AxisIO ay{};
+  ay.s_b = ay.d_b = 1;
+  ay.s_p = ay.d_p = d.KXP;
+  ay.s_o1 = ay.d_o1 = int64_t(d.KXP) * L1;
+  ay.nb = d.KX;
+  ay.n_in = L1;
+  ay.w0 = 0;
+  ay.n_out = L1;
+  launch_axis<double>(L1, -1, spec, spec, ay, L2, 1, twiddles<double>(c, L1), c->stream);
+  AxisIO az{};
+  az.s_b = az.d_b = 1;
+  az.s_p = az.d_p = int64_t(d.KXP) * L1;
+  az.s_o1 = az.d_o1 = d.KXP;
+  az.nb = d.KX;
+  az.n_in = L2;
+  az.w0 = 0;
+  az.n_out = L2;
+  launch_axis<double>(L2, -1, spec, spec, az, L1, 1, twiddles<double>(c, L2), c->stream);
+}
+
+void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_t n) {
+  hipLaunchKernelGGL(k_convert, dim3(nblk(n)), dim3(kThreads), 0, c->stream, in, out, n);
+  OSC_HIP(hipGetLastError());
+}
+
+template <typename T>
+const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) {
+  const SpecDims d = SpecDims::of(s.L);
+  const int64_t N = int64_t(s.L[0]) * s.L[1] * s.L[2];
+  std::string key(reinterpret_cast<const char*>(&s), sizeof(s));
+  GreenCache& gc = c->gcache;
+  if (gc.enable) {
+    auto it = gc.map.find(key);
+    if (it != gc.map.end()) {
+      GreenCache::Entry& e = *it->second;
+      e.last = ++gc.tick;
+      ++gc.hits;
+      if constexpr (sizeof(T) == 4) {
+        if (!e.f32.p) {
+          e.f32 = DevBuf(c->pool, sizeof(cpx<float>) * d.size());
+          convert_spectrum(c, e.f64.as<cpx<double>>(), e.f32.as<cpx<float>>(), d.size());
+          e.bytes += sizeof(cpx<float>) * d.size();
+          gc.bytes += sizeof(cpx<float>) * d.size();
+        }
+        return e.f32.as<cpx<float>>();
+      } else {
+        return e.f64.as<cpx<double>>();
+      }
+    }
+    ++gc.misses;
+  }
+  DevBuf g(c->pool, sizeof(double) * N);
+  build_green_real(c, s, g.as<double>());
+  DevBuf sp64(c->pool, sizeof(cpx<double>) * d.size());
+  spectrum_forward_full(c, g.as<const double>(), sp64.as<cpx<double>>(), d);
+  g.reset();
+  DevBuf sp32;
+  if constexpr (sizeof(T) == 4) {
+    sp32 = DevBuf(c->pool, sizeof(cpx<float>) * d.size());
+    convert_spectrum(c, sp64.as<cpx<double>>(), sp32.as<cpx<float>>(), d.size());
+  }
+  if (!gc.enable) {
+    if constexpr (sizeof(T) == 4) {
+      keepalive = std::move(sp32);
+      return keepalive.as<cpx<float>>();
+    } else {
+      keepalive = std::move(sp64);
+      return keepalive.as<cpx<double>>();
+    }
+  }
+  auto e = std::make_unique<GreenCache::Entry>();
+  e->bytes = sizeof(cpx<double>) * d.size() + (sp32.p ? sizeof(cpx<float>) * d.size() : 0);
+  e->f64 = std::move(sp64);
+  e->f32 = std::move(sp32);
+  e->last = ++gc.tick;
+  gc.bytes += e->bytes;
+  GreenCache::Entry* ep = e.get();
+  gc.map[key] = std::move(e);
+  // keep the entry being returned even if the budget is exceeded
+  if (gc.bytes > gc.max_bytes) {
+    ep->last = ~uint64_t(0);
+    gc.evict_to(gc.max_bytes);
+    ep->last = gc.tick;
+  }
+  if constexpr (sizeof(T) == 4) return ep->f32.as<cpx<float>>();
+  else return ep->f64.as<cpx<double>>();
+}
+template const cpx<double>* green_spectrum<double>(osc_ctx*, const GreenSpec&, DevBuf&);
+template const cpx<float>* green_spectrum<float>(osc_ctx*, const GreenSpec&, DevBuf&);
+
+// full upstream array F_s[g](k) = F_-[g](sigma_s k), sigma flipping axes with s=+1
+void half_to_upstream(osc_ctx* c, const cpx<double>* spec_dev, const SpecDims& d, const int s[3],
+                      double* full_host) {
+  std::vector<cpx<double>> h(d.size());
+  OSC_HIP(hipMemcpyAsync(h.data(), spec_dev, sizeof(cpx<double>) * d.size(),
+                         hipMemcpyDeviceToHost, c->stream));
+  OSC_HIP(hipStreamSynchronize(c->stream));
+  const int L0 = d.L[0], L1 = d.L[1], L2 = d.L[2];
+  cpx<double>* out = reinterpret_cast<cpx<double>*>(full_host);
+  for (int k = 0; k < L2; ++k)
+    for (int j = 0; j < L1; ++j)
+      for (int i = 0; i < L0; ++i) {
+        int a = s[0] > 0 ? (L0 - i) % L0 : i;
+        int b = s[1] > 0 ? (L1 - j) % L1 : j;
+        int e = s[2] > 0 ? (L2 - k) % L2 : k;
+        cpx<double> v;
+        if (a <= L0 / 2) {
+          v = h[a + int64_t(d.KXP) * (b + int64_t(L1) * e)];
+        } else {
+          v = h[(L0 - a) + int64_t(d.KXP) * ((L1 - b) % L1 + int64_t(L1) * ((L2 - e) % L2))];
+          v.y = -v.y;
+        }
+        out[i + int64_t(L0) * (j + int64_t(L1) * k)] = v;
+      }
+}
+
+void upstream_to_half(osc_ctx* c, const double* full_host, const SpecDims& d, const int s[3],
+                      cpx<double>* spec_dev) {
+  const int L0 = d.L[0], L1 = d.L[1], L2 = d.L[2];
+  std::vector<cpx<double>> h(d.size(), mk(0.0, 0.0));
+  const cpx<double>* in = reinterpret_cast<const cpx<double>*>(full_host);
+  for (int k = 0; k < L2; ++k)
+    for (int j = 0; j < L1; ++j)
+      for (int i = 0; i < d.KX; ++i) {
+        int a = s[0] > 0 ? (L0 - i) % L0 : i;
+        int b = s[1] > 0 ? (L1 - j) % L1 : j;
+        int e = s[2] > 0 ? (L2 - k) % L2 : k;
+        h[i + int64_t(d.KXP) * (j + int64_t(L1) * k)] = in[a + int64_t(L0) * (b + int64_t(L1) * e)];
+      }
+  OSC_HIP(hipMemcpyAsync(spec_dev, h.data(), sizeof(cpx<double>) * d.size(),
+                         hipMemcpyHostToDevice, c->stream));
+  OSC_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace osc

@@ -1,0 +1,195 @@
+// Context, caches and internal entry points of the library.
+#pragma once
+#include <list>
+#include <map>
+#include <memory>
+
+#include "fft_launch.h"
+
+namespace osc {
+
+// dimensions of a doubled-grid (or period-L) half spectrum
+struct SpecDims {
+  int L[3];
+  int KX;   // L0/2 + 1
+  int KXP;  // padded row length
+  static SpecDims of(const int L_[3]) {
+    SpecDims d;
+    for (int i = 0; i < 3; ++i) d.L[i] = L_[i];
+    d.KX = L_[0] / 2 + 1;
+    d.KXP = (d.KX + 15) / 16 * 16;
+    return d;
+  }
+  int64_t size() const { return int64_t(KXP) * L[1] * L[2]; }
+};
+
+// Everything that determines one real-space Green array (all fields are part
+// of the cache key; construct with make_green_spec so padding is zero).
+enum GreenKind : int {
+  GK_OFFSET = 0,      // osc_get_cgrn_freespace (open_spacecharge_mod.f90:633-707)
+  GK_LEGACY = 1,      // osc_getgrnfree loop (core:454-497)
+  GK_IMG_SHIFT = 2,   // osc_getgrnimageshift loop (core:1135-1178)
+  GK_IMG_CONVCORR = 3,// osc_getgrnimageconvcorr loop (core:1185-1230)
+  GK_PIPE = 4         // osc_getgrnpipe loops (core:652-713)
+};
+struct GreenSpec {
+  int32_t kind, icomp, igf, term;
+  int32_t L[3];
+  int32_t lo[3];      // lower index bound of the Green array (legacy / pipe)
+  int32_t shift[3];   // iperiod/2-(ipad+1)/2 (legacy / pipe)
+  int32_t nlo3;
+  int32_t out_shift_z;  // circular shift applied to the output z index
+  int32_t pad_;
+  double delta[3], gamma, offset[3], zshft, umin[3], a, b;
+};
+GreenSpec make_green_spec();
+
+class GreenCache {
+ public:
+  struct Entry {
+    DevBuf f64, f32;
+    size_t bytes = 0;
+    uint64_t last = 0;
+  };
+  bool enable = true;
+  size_t max_bytes = size_t(8) << 30;
+  size_t bytes = 0;
+  uint64_t tick = 0;
+  std::unordered_map<std::string, std::unique_ptr<Entry>> map;
+  uint64_t hits = 0, misses = 0;
+  void clear() {
+    map.clear();
+    bytes = 0;
+  }
+  void evict_to(size_t budget);
+};
+
+// pipe Green spectra (the module globals cgrn1..4 of core:22-23)
+struct PipeState {
+  bool alloc = false, built = false;
+  int nlo[3], nhi[3], npad[3], L[3];
+  double a = 0, b = 0, gamma = 0, delta[3] = {0, 0, 0}, umin[3] = {0, 0, 0};
+  DevBuf spec64[4], spec32[4];
+};
+
+}  // namespace osc
+
+struct osc_ctx {
+  int device = 0;
+  hipStream_t own = nullptr, stream = nullptr;
+  bool async = false;
+  int prec = OSC_PREC_FP64;
+  osc::DevPool pool;
+  std::string last_error;
+  std::map<int, void*> tw64, tw32;
+  osc::GreenCache gcache;
+  osc::PipeState pipe;
+  void* counters = nullptr;  // small device scratch for counters (persistent)
+  void* host_pinned = nullptr;
+};
+
+namespace osc {
+
+// current device guard
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    (void)hipGetDevice(&prev);
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    if (prev >= 0 && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+template <typename T>
+const cpx<T>* twiddles(osc_ctx* c, int L);
+
+// Staging of caller arrays that may be host or device memory.
+class Staging {
+ public:
+  explicit Staging(osc_ctx* c) : c_(c) {}
+  template <typename U>
+  const U* in(const U* p, size_t n);
+  template <typename U>
+  U* out(U* p, size_t n, bool copy_in = false);
+  void finish();  // copies outputs back; synchronises unless fully device-resident + async
+  bool any_host() const { return any_host_; }
+
+ private:
+  struct Pending {
+    void* host;
+    void* dev;
+    size_t bytes;
+  };
+  osc_ctx* c_;
+  bool any_host_ = false;
+  std::vector<Pending> outs_;
+  std::vector<DevBuf> bufs_;
+};
+
+bool is_device_ptr(const void* p);
+
+// ---- green.hip --------------------------------------------------------------
+void build_green_real(osc_ctx* c, const GreenSpec& s, double* g);  // (L0,L1,L2) real
+// half spectrum of a real-space Green array in precision T; cached by spec
+template <typename T>
+const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive);
+// forward R2C of a real (L0,L1,L2) array into a half spectrum (FP64)
+void spectrum_forward_full(osc_ctx* c, const double* g, cpx<double>* spec, const SpecDims& d);
+void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_t n);
+// download a half spectrum and expand it to the upstream full-array
+// convention F_s[g] (sign s[d] on axis d)
+void half_to_upstream(osc_ctx* c, const cpx<double>* spec_dev, const SpecDims& d, const int s[3],
+                      double* full_host);
+void upstream_to_half(osc_ctx* c, const double* full_host, const SpecDims& d, const int s[3],
+                      cpx<double>* spec_dev);
+
+// ---- solve.hip ----------------------------------------------------------------
+struct OutSpec {
+  int nt = 0;
+  struct {
+    const void* g;
+    double sgn;
+    int mir;
+  } t[kMaxTerm];
+  Epilogue ep;
+};
+template <typename T>
+void conv_solve(osc_ctx* c, const double* rho, const int n[3], const SpecDims& d, const int win[3],
+                const std::vector<OutSpec>& outs);
+void fill_f64(osc_ctx* c, double* p, int64_t n, double v);
+void fill_u32(osc_ctx* c, uint32_t* p, int64_t n, uint32_t v);
+void fd_gradient(osc_ctx* c, const double* phi, const int n[3], const double delta[3], double gam,
+                 double* ef);
+void bfield_core(osc_ctx* c, const double* ef, int64_t n, double gam, double* bf);
+void flip_z(osc_ctx* c, double* rho, const int n[3]);
+
+// ---- deposit.hip ----------------------------------------------------------------
+struct DepositArgs {
+  int64_t n;
+  const double *x, *y, *z, *q;
+  int nlo[3], nn[3];
+  double mn[3], delta[3];
+  double q_each;        // uniform charge per particle (q == nullptr, charge mode)
+  int harness;          // 1: depose_rho_scalar rules (unit weights, scale at end)
+  double scale_after;   // harness chrgpermacro
+};
+// rho (nx,ny,nz) on device; returns dropped count (synchronises)
+int64_t deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho);
+void bbox(osc_ctx* c, int64_t n, const double* x, const double* y, const double* z, double lo[3],
+          double hi[3]);
+double device_sum(osc_ctx* c, const double* v, int64_t n);
+
+// ---- gather.hip -------------------------------------------------------------------
+struct InterpArgs {
+  int nlo[3], nhi[3], nn[3];
+  double mn[3], delta[3], hi[3];
+};
+int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
+                    int64_t n, const double* x, const double* y, const double* z, double* E,
+                    double* B, int8_t* clamped);
+
+}  // namespace osc

@@ -1,0 +1,971 @@
+/*
+ * osc_oracle.c -- CPU restatement of the upstream OpenSC space-charge path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle the HIP library is
+ * checked against, and a CPU baseline; it is built by oracle/Makefile into
+ * oracle/liboscoracle.so and is loaded only by tests/, smoke() and bench.py's
+ * cpu_baseline leg.  The product path never links it.
+ *
+ * Each function restates the upstream routine cited beside it, in the same
+ * floating-point operation order (left-to-right Fortran evaluation, no FMA
+ * contraction: build with -ffp-contract=off), so that Green functions,
+ * deposition and interpolation agree bit-for-bit with the compiled upstream
+ * library and FFT-based fields agree to round-off.  The FFT is this file's own
+ * mixed-radix transform (any exact DFT agrees with the upstream Singleton FFT
+ * to round-off; only its sign convention and normalisation are restated).
+ *
+ * Parity status of every function is pinned by tests/test_oracle_*.py
+ * (against the compiled upstream library, numpy.fft, brute force, quadrature
+ * and closed forms).
+ */
+#include "osc_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* 299792458.0 is a default-real literal upstream: float(299792458) = 299792448 */
+const double oc_clight_sp = 299792448.0;
+const double oc_clight_dp = 299792458.0;
+/* fpei = 299792458.0**2*1.00000000055d-7, square taken in REAL(4)
+   (open_spacecharge_mod.f90:547) */
+const double oc_fpei_solver2 = 8987551502.2918644;
+/* fpei = 299792458.d0**2*1.d-7 (core:515,1250) */
+const double oc_fpei_conv3d = 299792458.0 * 299792458.0 * 1e-7;
+/* fpei = 299792458.**2*1.e-7, all REAL(4) (core:783) */
+const double oc_fpei_pipe = 8987551744.0;
+
+typedef struct {
+  double re, im;
+} cplx;
+
+#define IDX3(i, j, k, n1, n2) ((int64_t)(i) + (int64_t)(n1) * ((int64_t)(j) + (int64_t)(n2) * (int64_t)(k)))
+
+static inline cplx cmul(cplx a, cplx b) {
+  cplx r;
+  r.re = a.re * b.re - a.im * b.im;
+  r.im = a.re * b.im + a.im * b.re;
+  return r;
+}
+static inline cplx cadd(cplx a, cplx b) {
+  cplx r = {a.re + b.re, a.im + b.im};
+  return r;
+}
+
+/* ========================================================================== */
+/* FFT                                                                         */
+/* ========================================================================== */
+
+/* prime factors of n (ascending); -1 if a factor exceeds 23 (the upstream
+   Singleton routine's limit, fast_fourier_am.f90:147-161) */
+static int factorize(int n, int *fac) {
+  int nf = 0;
+  for (int p = 2; n > 1 && p <= 23; ++p)
+    while (n % p == 0) {
+      fac[nf++] = p;
+      n /= p;
+    }
+  return n == 1 ? nf : -1;
+}
+
+/* recursive mixed-radix decimation in time.  W[x] = exp(sign*2*pi*i*x/N) for
+   the top-level length N; a sub-transform of length n uses W[x*(N/n)]. */
+static void fft_rec(const cplx *in, int64_t istride, cplx *out, int n, const int *fac,
+                    const cplx *W, int N) {
+  if (n == 1) {
+    out[0] = in[0];
+    return;
+  }
+  int p = fac[0], m = n / p, ws = N / n;
+  for (int r = 0; r < p; ++r) fft_rec(in + r * istride, istride * p, out + r * m, m, fac + 1, W, N);
+  cplx t[23], s[23];
+  for (int k = 0; k < m; ++k) {
+    for (int r = 0; r < p; ++r) {
+      cplx v = out[r * m + k];
+      t[r] = (r == 0 || k == 0) ? v : cmul(v, W[((int64_t)r * k * ws) % N]);
+    }
+    for (int q = 0; q < p; ++q) {
+      cplx acc = t[0];
+      for (int r = 1; r < p; ++r) acc = cadd(acc, cmul(t[r], W[((int64_t)r * q * m * ws) % N]));
+      s[q] = acc;
+    }
+    for (int q = 0; q < p; ++q) out[k + q * m] = s[q];
+  }
+}
+
+static void twiddles(cplx *W, int n, int sign) {
+  const double twopi = 6.283185307179586476925286766559;
+  for (int x = 0; x < n; ++x) {
+    double ang = twopi * (double)x / (double)n;
+    W[x].re = cos(ang);
+    W[x].im = sign * sin(ang);
+  }
+}
+
+/* batched 1-D transforms along one axis of an (n1,n2,n3) array */
+static int fft_axis(cplx *a, int n1, int n2, int n3, int axis, int sign) {
+  int n = axis == 0 ? n1 : axis == 1 ? n2 : n3;
+  int fac[40];
+  if (n == 1) return 0;
+  if (factorize(n, fac) < 0) return -1;
+  cplx *W = malloc(sizeof(cplx) * n), *lin = malloc(sizeof(cplx) * n), *lout = malloc(sizeof(cplx) * n);
+  twiddles(W, n, sign);
+  int64_t stride = axis == 0 ? 1 : axis == 1 ? n1 : (int64_t)n1 * n2;
+  int64_t nlines = (int64_t)n1 * n2 * n3 / n;
+  for (int64_t l = 0; l < nlines; ++l) {
+    int64_t base;
+    if (axis == 0) base = l * n1;
+    else if (axis == 1) base = (l % n1) + (l / n1) * (int64_t)n1 * n2;
+    else base = l;
+    for (int j = 0; j < n; ++j) lin[j] = a[base + j * stride];
+    fft_rec(lin, 1, lout, n, fac, W, n);
+    for (int j = 0; j < n; ++j) a[base + j * stride] = lout[j];
+  }
+  free(W);
+  free(lin);
+  free(lout);
+  return 0;
+}
+
+/* ccfft3d (fft_interface_mod.f90:96-143): per-axis sign idir(d), unnormalised;
+   idir = (0,0,0) is the identity (upstream returns early). */
+int oc_ccfft3d(double *a, int n1, int n2, int n3, const int idir[3]) {
+  if (idir[0] == 0 && idir[1] == 0 && idir[2] == 0) return 0;
+  cplx *c = (cplx *)a;
+  if (fft_axis(c, n1, n2, n3, 0, idir[0] >= 0 ? 1 : -1)) return -1;
+  if (fft_axis(c, n1, n2, n3, 1, idir[1] >= 0 ? 1 : -1)) return -1;
+  if (fft_axis(c, n1, n2, n3, 2, idir[2] >= 0 ? 1 : -1)) return -1;
+  return 0;
+}
+
+/* ========================================================================== */
+/* Deposition                                                                  */
+/* ========================================================================== */
+
+/* deposit_particles (open_spacecharge_mod.f90:156-288) */
+int oc_deposit_particles(int64_t n, const double *x, const double *y, const double *z,
+                         const double *q, int has_total, double total_charge, int resize,
+                         const int nlo[3], const int nhi[3], double mn[3], double mx[3],
+                         double delta[3], double *charge, double *rho, int64_t *n_dropped) {
+  const double *c[3] = {x, y, z};
+  if (resize) { /* :176-189 */
+    double lo[3], hi[3], d[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = c[a][0];
+      hi[a] = c[a][0];
+      for (int64_t i = 1; i < n; ++i) {
+        if (c[a][i] < lo[a]) lo[a] = c[a][i];
+        if (c[a][i] > hi[a]) hi[a] = c[a][i];
+      }
+      d[a] = (hi[a] - lo[a]) / (double)(nhi[a] - nlo[a]);
+      lo[a] = lo[a] - 1.0e-6 * d[a];
+      hi[a] = hi[a] + 1.0e-6 * d[a];
+      d[a] = (hi[a] - lo[a]) / (double)(nhi[a] - nlo[a]);
+      mn[a] = lo[a];
+      mx[a] = hi[a];
+      delta[a] = d[a];
+    }
+  }
+  int nx = nhi[0] - nlo[0] + 1, ny = nhi[1] - nlo[1] + 1, nz = nhi[2] - nlo[2] + 1;
+  double dx = delta[0], dy = delta[1], dz = delta[2];
+  double dxi = 1.0 / dx, dyi = 1.0 / dy, dzi = 1.0 / dz;
+  double xmin = mn[0], ymin = mn[1], zmin = mn[2];
+  double charge1 = 0.0;
+  if (q) { /* :244-252 */
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s = s + q[i];
+    *charge = s;
+  } else if (has_total) {
+    *charge = total_charge;
+    charge1 = total_charge / (double)n;
+  } else {
+    return -1;
+  }
+  memset(rho, 0, sizeof(double) * (size_t)nx * ny * nz);
+  int64_t dropped = 0;
+  for (int64_t p = 0; p < n; ++p) { /* :257-283 */
+    int ip = (int)floor((x[p] - xmin) * dxi + 1.0);
+    int jp = (int)floor((y[p] - ymin) * dyi + 1.0);
+    int kp = (int)floor((z[p] - zmin) * dzi + 1.0);
+    double ab = ((xmin - x[p]) + ip * dx) * dxi;
+    double de = ((ymin - y[p]) + jp * dy) * dyi;
+    double gh = ((zmin - z[p]) + kp * dz) * dzi;
+    /* upstream: no bounds check; here particles off the mesh are skipped */
+    if (ip < 1 || jp < 1 || kp < 1 || ip + 1 > nx || jp + 1 > ny || kp + 1 > nz) {
+      ++dropped;
+      continue;
+    }
+    if (q) charge1 = q[p];
+    int i0 = ip - 1, j0 = jp - 1, k0 = kp - 1;
+#define R(i, j, k) rho[IDX3(i, j, k, nx, ny)]
+    R(i0, j0, k0) = R(i0, j0, k0) + ab * de * gh * charge1;
+    R(i0, j0 + 1, k0) = R(i0, j0 + 1, k0) + ab * (1. - de) * gh * charge1;
+    R(i0, j0 + 1, k0 + 1) = R(i0, j0 + 1, k0 + 1) + ab * (1. - de) * (1. - gh) * charge1;
+    R(i0, j0, k0 + 1) = R(i0, j0, k0 + 1) + ab * de * (1. - gh) * charge1;
+    R(i0 + 1, j0, k0 + 1) = R(i0 + 1, j0, k0 + 1) + (1. - ab) * de * (1. - gh) * charge1;
+    R(i0 + 1, j0 + 1, k0 + 1) = R(i0 + 1, j0 + 1, k0 + 1) + (1. - ab) * (1. - de) * (1. - gh) * charge1;
+    R(i0 + 1, j0 + 1, k0) = R(i0 + 1, j0 + 1, k0) + (1. - ab) * (1. - de) * gh * charge1;
+    R(i0 + 1, j0, k0) = R(i0 + 1, j0, k0) + (1. - ab) * de * gh * charge1;
+#undef R
+  }
+  if (n_dropped) *n_dropped = dropped;
+  return 0;
+}
+
+static double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
+
+/* get_mesh_quantities (test_mod.f90:288-358) with getbeamboundingbox (:366-407) */
+void oc_get_mesh_quantities(int64_t n, const double *x, const double *y, const double *z,
+                            const int nlo[3], const int nhi[3], int isetzmin,
+                            double delta[3], double umin[3]) {
+  const double *c[3] = {x, y, z};
+  double sml[3], big[3];
+  const double eps = 1.e-13;
+  for (int a = 0; a < 3; ++a) {
+    double mnv = -9999999., mxv = -9999999.; /* upstream sentinel when n == 0 */
+    if (n > 0) {
+      double lo = c[a][0], hi = c[a][0];
+      for (int64_t i = 1; i < n; ++i) {
+        if (c[a][i] < lo) lo = c[a][i];
+        if (c[a][i] > hi) hi = c[a][i];
+      }
+      mnv = -lo;
+      mxv = hi;
+    }
+    sml[a] = -mnv;
+    big[a] = mxv;
+    big[a] = big[a] * (1.0 + fsign(1.0, big[a]) * eps);
+    sml[a] = sml[a] * (1.0 - fsign(1.0, sml[a]) * eps);
+  }
+  for (int a = 0; a < 3; ++a) {
+    int na = nhi[a] - nlo[a] + 1;
+    delta[a] = (big[a] - sml[a]) / (double)(na - 3);
+    umin[a] = sml[a] - delta[a];
+  }
+  if (isetzmin == 1) {
+    int nz = nhi[2] - nlo[2] + 1;
+    double zsml = 1.e-9;
+    delta[2] = (big[2] - zsml) / (double)(nz - 1);
+    umin[2] = zsml;
+  }
+}
+
+/* depose_rho_scalar (test_mod.f90:410-474), nlogbl = nlo */
+int64_t oc_depose_rho_scalar(int64_t n, const double *x, const double *y, const double *z,
+                             const int nlo[3], const int nhi[3], double chrgpermacro,
+                             const double delta[3], const double umin[3], double *rho) {
+  int nx = nhi[0] - nlo[0] + 1, ny = nhi[1] - nlo[1] + 1, nz = nhi[2] - nlo[2] + 1;
+  int ilo = nlo[0], jlo = nlo[1], klo = nlo[2];
+  int ihi = nx, jhi = ny, khi = nz; /* size(rho, d), as upstream :432 */
+  double dx = delta[0], dy = delta[1], dz = delta[2];
+  double xmin = umin[0], ymin = umin[1], zmin = umin[2];
+  double dxi = 1.0 / dx, dyi = 1.0 / dy, dzi = 1.0 / dz;
+  int64_t ifail = 0;
+  memset(rho, 0, sizeof(double) * (size_t)nx * ny * nz);
+  for (int64_t p = 0; p < n; ++p) {
+    int ip = (int)floor((x[p] - xmin) * dxi + (double)ilo);
+    int jp = (int)floor((y[p] - ymin) * dyi + (double)jlo);
+    int kp = (int)floor((z[p] - zmin) * dzi + (double)klo);
+    double ab = ((xmin - x[p]) + (ip - ilo + 1) * dx) * dxi;
+    double de = ((ymin - y[p]) + (jp - jlo + 1) * dy) * dyi;
+    double gh = ((zmin - z[p]) + (kp - klo + 1) * dz) * dzi;
+    /* upstream range check (:457) allows ip == ihi and then writes ip+1; the
+       node ip+1 must exist here, so that case is also counted as a failure */
+    if (ip < ilo || jp < jlo || kp < klo || ip > ihi || jp > jhi || kp > khi ||
+        ip + 1 > ihi + ilo - 1 || jp + 1 > jhi + jlo - 1 || kp + 1 > khi + klo - 1) {
+      ++ifail;
+      continue;
+    }
+    int i0 = ip - ilo, j0 = jp - jlo, k0 = kp - klo;
+#define R(i, j, k) rho[IDX3(i, j, k, nx, ny)]
+    R(i0, j0, k0) = R(i0, j0, k0) + ab * de * gh;
+    R(i0, j0 + 1, k0) = R(i0, j0 + 1, k0) + ab * (1. - de) * gh;
+    R(i0, j0 + 1, k0 + 1) = R(i0, j0 + 1, k0 + 1) + ab * (1. - de) * (1. - gh);
+    R(i0, j0, k0 + 1) = R(i0, j0, k0 + 1) + ab * de * (1. - gh);
+    R(i0 + 1, j0, k0 + 1) = R(i0 + 1, j0, k0 + 1) + (1. - ab) * de * (1. - gh);
+    R(i0 + 1, j0 + 1, k0 + 1) = R(i0 + 1, j0 + 1, k0 + 1) + (1. - ab) * (1. - de) * (1. - gh);
+    R(i0 + 1, j0 + 1, k0) = R(i0 + 1, j0 + 1, k0) + (1. - ab) * (1. - de) * gh;
+    R(i0 + 1, j0, k0) = R(i0 + 1, j0, k0) + (1. - ab) * de * gh;
+#undef R
+  }
+  int64_t nt = (int64_t)nx * ny * nz;
+  for (int64_t i = 0; i < nt; ++i) rho[i] = rho[i] * chrgpermacro;
+  return ifail;
+}
+
+/* ========================================================================== */
+/* IGF primitives                                                              */
+/* ========================================================================== */
+
+/* xlafun2 (open_spacecharge_mod.f90:721-726) */
+double oc_xlafun2(double x, double y, double z) {
+  double r = sqrt(x * x + y * y + z * z);
+  return x * atan((y * z) / (r * x)) - z * log(r + y) + y * log((r - z) / (r + z)) / 2.0;
+}
+
+/* lafun2 (open_spacecharge_mod.f90:741-747) */
+double oc_lafun2(double x, double y, double z) {
+  double r = sqrt(x * x + y * y + z * z);
+  return -(z * z * atan(x * y / (z * r)) / 2.0) - y * y * atan(x * z / (y * r)) / 2.0 -
+         x * x * atan(y * z / (x * r)) / 2.0 + y * z * log(x + r) + x * z * log(y + r) +
+         x * y * log(z + r);
+}
+
+/* lafun (core:229-238) */
+double oc_lafun(double x, double y, double z) {
+  double r = sqrt(x * x + y * y + z * z);
+  return -(0.5 * (z * z) * atan(x * y / (z * r))) - 0.5 * (y * y) * atan(x * z / (y * r)) -
+         0.5 * (x * x) * atan(y * z / (x * r)) + y * z * log(x + r) + x * z * log(y + r) +
+         x * y * log(z + r);
+}
+
+/* xlafun (core:377-392) */
+double oc_xlafun(double x, double y, double z) {
+  double r = sqrt(x * x + y * y + z * z);
+  double res = z - x * atan(z / x) + x * atan(y * z / (x * r));
+  if (y + r != 0) res = res - z * log(y + r);
+  if (z + r != 0) res = res - y * log(z + r);
+  return res;
+}
+
+/* ylafun (core:397-407) */
+double oc_ylafun(double x, double y, double z) {
+  double r = sqrt(x * x + y * y + z * z);
+  double res = x - y * atan(x / y) + y * atan(z * x / (y * r));
+  if (z + r != 0) res = res - x * log(z + r);
+  if (x + r != 0) res = res - z * log(x + r);
+  return res;
+}
+
+/* zlafun (core:412-428) */
+double oc_zlafun(double x, double y, double z) {
+  double r = sqrt(x * x + y * y + z * z);
+  double res = y - z * atan(y / z) + z * atan(x * y / (z * r));
+  if (x + r != 0) res = res - y * log(x + r);
+  if (y + r != 0) res = res - x * log(y + r);
+  return res;
+}
+
+/* coulombfun (core:189-200) */
+static double coulombfun(double u, double v, double w, double gam) {
+  if (u == 0.0 && v == 0.0 && w == 0.0) return 0.0;
+  return 1.0 / sqrt(u * u + v * v + (gam * w) * (gam * w));
+}
+
+typedef double (*lafun_t)(double, double, double);
+
+/* igfcoulombfun / igfexfun / igfeyfun / igfezfun (core:205-372) */
+static double igf8(lafun_t f, double x1, double x2, double y1, double y2, double z1, double z2) {
+  return f(x2, y2, z2) - f(x1, y2, z2) - f(x2, y1, z2) - f(x2, y2, z1) - f(x1, y1, z1) +
+         f(x1, y1, z2) + f(x1, y2, z1) + f(x2, y1, z1);
+}
+
+double oc_igf(int igfflag, int icomp, double u, double v, double w, double gam, double dx,
+              double dy, double dz) {
+  if (igfflag == 0) return icomp == 0 ? coulombfun(u, v, w, gam) : 0.0;
+  double x1 = u - 0.5 * dx, x2 = u + 0.5 * dx;
+  double y1 = v - 0.5 * dy, y2 = v + 0.5 * dy;
+  double z1 = (w - 0.5 * dz) * gam, z2 = (w + 0.5 * dz) * gam;
+  if (icomp == 0) return igf8(oc_lafun, x1, x2, y1, y2, z1, z2) / (dx * dy * dz * gam);
+  double res;
+  if (x1 < 0.0 && x2 > 0.0 && y1 < 0.0 && y2 > 0.0 && z1 < 0.0 && z2 > 0.0) {
+    res = 0.0;
+  } else {
+    lafun_t f = icomp == 1 ? oc_xlafun : icomp == 2 ? oc_ylafun : oc_zlafun;
+    res = igf8(f, x1, x2, y1, y2, z1, z2);
+  }
+  if (icomp == 3) return res / (dx * dy * dz * gam);
+  return res / (dx * dy * dz);
+}
+
+/* rfun (core:723-755) */
+double oc_rfun(double u, double v, double w, double gam, double a, double b, double hz, int i,
+               int j) {
+  const double pi = acos(-1.0);
+  double ainv = 1.0 / a, binv = 1.0 / b;
+  double piainv = pi * ainv, pibinv = pi * binv;
+  double res = 0.0;
+  for (int m = 1; m <= 5; ++m)
+    for (int n = 1; n <= 5; ++n) {
+      double km = m * pi * ainv, kn = n * pi * binv;
+      double kapmn = sqrt(km * km + kn * kn);
+      double zfun = (exp(-(kapmn * fabs(gam * w - hz))) - 2.0 * exp(-(kapmn * fabs(gam * w))) +
+                     exp(-(kapmn * fabs(gam * w + hz)))) /
+                    (hz * hz * (kapmn * kapmn));
+      if (w == 0) zfun = zfun + 2.0 / (hz * kapmn);
+      int im = 1, jn = 1;
+      for (int t = 0; t < m; ++t) im *= i;
+      for (int t = 0; t < n; ++t) jn *= j;
+      double term = (double)(im * jn) * cos(m * u * piainv) * cos(n * v * pibinv) * zfun / kapmn;
+      res = res + term;
+    }
+  res = res * 2.0 * pi * ainv * binv;
+  return res;
+}
+
+/* ========================================================================== */
+/* Green-function builders                                                     */
+/* ========================================================================== */
+
+/* osc_get_cgrn_freespace (open_spacecharge_mod.f90:633-707) */
+void oc_get_cgrn_freespace(double *cgrn, int isize, int jsize, int ksize,
+                           const double delta[3], double gamma, int icomp,
+                           const double *offset) {
+  double dx = delta[0], dy = delta[1], dz = delta[2] * gamma;
+  double factor = (icomp == 1 || icomp == 2) ? gamma / (dx * dy * dz) : 1.0 / (dx * dy * dz);
+  double umin = (0.5 - (double)(isize / 2)) * dx;
+  double vmin = (0.5 - (double)(jsize / 2)) * dy;
+  double wmin = (0.5 - (double)(ksize / 2)) * dz;
+  if (offset) {
+    umin = umin + offset[0];
+    vmin = vmin + offset[1];
+    wmin = wmin + offset[2] * gamma;
+  }
+  cplx *g = (cplx *)cgrn;
+  for (int k = 0; k < ksize; ++k) {
+    double w = k * dz + wmin;
+    for (int j = 0; j < jsize; ++j) {
+      double v = j * dy + vmin;
+      for (int i = 0; i < isize; ++i) {
+        double u = i * dx + umin, gval = 0.0;
+        if (icomp == 0) gval = oc_lafun2(u, v, w) * factor;
+        if (icomp == 1) gval = oc_xlafun2(u, v, w) * factor;
+        if (icomp == 2) gval = oc_xlafun2(v, w, u) * factor;
+        if (icomp == 3) gval = oc_xlafun2(w, u, v) * factor;
+        g[IDX3(i, j, k, isize, jsize)].re = gval;
+        g[IDX3(i, j, k, isize, jsize)].im = 0.0;
+      }
+    }
+  }
+  /* 8-corner difference over 1:size-1 (:697-705); array semantics: RHS first */
+  int64_t nt = (int64_t)isize * jsize * ksize;
+  double *t = malloc(sizeof(double) * nt);
+  for (int64_t i = 0; i < nt; ++i) t[i] = g[i].re;
+#define T(i, j, k) t[IDX3(i, j, k, isize, jsize)]
+  for (int k = 0; k < ksize - 1; ++k)
+    for (int j = 0; j < jsize - 1; ++j)
+      for (int i = 0; i < isize - 1; ++i) {
+        double s = T(i + 1, j + 1, k + 1) - T(i, j + 1, k + 1) - T(i + 1, j, k + 1) -
+                   T(i + 1, j + 1, k) - T(i, j, k) + T(i, j, k + 1) + T(i, j + 1, k) +
+                   T(i + 1, j, k);
+        g[IDX3(i, j, k, isize, jsize)].re = s;
+      }
+#undef T
+  free(t);
+}
+
+/* shared loop of osc_getgrnfree / osc_getgrnimageshift / osc_getgrnimageconvcorr.
+   mode 0: w = kp*dz (+zshft); mode 1: convcorr, w = k*dz + 2*umin3 - 2*nlo3*dz */
+static void grn_legacy(double *cgrn, const int lo[3], const int per[3], const int npad[3],
+                       const double delta[3], double gamma, int icomp, int igfflag, double zshft,
+                       int convcorr, const double *umin, const int *nlo, const int fftdir[3]) {
+  double dx = delta[0], dy = delta[1], dz = delta[2];
+  int ishift = per[0] / 2 - (npad[0] + 1) / 2;
+  int jshift = per[1] / 2 - (npad[1] + 1) / 2;
+  int kshift = per[2] / 2 - (npad[2] + 1) / 2;
+  cplx *g = (cplx *)cgrn;
+  for (int kk = 0; kk < per[2]; ++kk) {
+    int k = lo[2] + kk;
+    double w;
+    if (convcorr) {
+      int kp = k;
+      w = kp * dz + 2.0 * umin[2] - 2.0 * nlo[2] * delta[2];
+    } else {
+      int kp = lo[2] + (k - lo[2] + kshift) % per[2];
+      w = kp * dz + zshft;
+    }
+    for (int jj = 0; jj < per[1]; ++jj) {
+      int jp = lo[1] + (jj + jshift) % per[1];
+      double v = jp * dy;
+      for (int ii = 0; ii < per[0]; ++ii) {
+        int ip = lo[0] + (ii + ishift) % per[0];
+        double u = ip * dx;
+        double gval = oc_igf(igfflag, icomp, u, v, w, gamma, dx, dy, dz);
+        g[IDX3(ii, jj, kk, per[0], per[1])].re = gval;
+        g[IDX3(ii, jj, kk, per[0], per[1])].im = 0.0;
+      }
+    }
+  }
+  oc_ccfft3d(cgrn, per[0], per[1], per[2], fftdir);
+}
+
+static void legacy_bounds(const int nlo[3], const int nhi[3], const int npad[3], int lo[3],
+                          int per[3]) {
+  for (int a = 0; a < 3; ++a) { /* osc_alloc_freespace_array (core:151-182) */
+    lo[a] = nlo[a] - nhi[a];
+    int hi = nhi[a] - nlo[a] + npad[a];
+    per[a] = hi - lo[a] + 1;
+  }
+}
+
+void oc_getgrnfree(double *cgrn, const int nlo[3], const int nhi[3], const int npad[3],
+                   const double delta[3], double gamma, int icomp, int igfflag) {
+  int lo[3], per[3];
+  const int dir[3] = {1, 1, 1};
+  legacy_bounds(nlo, nhi, npad, lo, per);
+  grn_legacy(cgrn, lo, per, npad, delta, gamma, icomp, igfflag, 0.0, 0, NULL, NULL, dir);
+}
+
+void oc_getgrnimageshift(double *cgrn, const int nlo[3], const int nhi[3], const int npad[3],
+                         const double delta[3], double gamma, double zshft, int icomp,
+                         int igfflag) {
+  int lo[3], per[3];
+  const int dir[3] = {1, 1, 1};
+  legacy_bounds(nlo, nhi, npad, lo, per);
+  grn_legacy(cgrn, lo, per, npad, delta, gamma, icomp, igfflag, zshft, 0, NULL, NULL, dir);
+}
+
+void oc_getgrnimageconvcorr(double *cgrn, const int nlo[3], const int nhi[3],
+                            const int npad[3], const double umin[3], const double delta[3],
+                            double gamma, int icomp, int igfflag) {
+  int lo[3], per[3];
+  const int dir[3] = {1, 1, -1};
+  legacy_bounds(nlo, nhi, npad, lo, per); /* x, y as cgrn1 */
+  lo[2] = 2 * nlo[2];                       /* osc_alloc_image_array (core:1095-1128) */
+  per[2] = 2 * nhi[2] + npad[2] - lo[2] + 1;
+  grn_legacy(cgrn, lo, per, npad, delta, gamma, icomp, igfflag, 0.0, 1, umin, nlo, dir);
+}
+
+/* osc_getgrnpipe (core:632-715) with bounds of osc_alloc_rectpipe_arrays (core:876-898) */
+void oc_getgrnpipe(double *g[4], const int nlo[3], const int nhi[3], const int npad[3],
+                   double gamma, double a, double b, const double delta[3],
+                   const double umin[3]) {
+  double hx = delta[0], hy = delta[1], hz = delta[2];
+  double xmin = umin[0], ymin = umin[1];
+  int g1lo[3], per[3];
+  for (int d = 0; d < 3; ++d) {
+    g1lo[d] = nlo[d] - nhi[d];
+    per[d] = (nhi[d] - nlo[d] + npad[d]) - g1lo[d] + 1;
+  }
+  int ishift = per[0] / 2 - (npad[0] + 1) / 2;
+  int jshift = per[1] / 2 - (npad[1] + 1) / 2;
+  int kshift = per[2] / 2 - (npad[2] + 1) / 2;
+  const int sx[4] = {1, 1, -1, -1}, sy[4] = {1, -1, 1, -1};
+  for (int t = 0; t < 4; ++t) {
+    cplx *c = (cplx *)g[t];
+    for (int kk = 0; kk < per[2]; ++kk) {
+      int kp = g1lo[2] + (kk + kshift) % per[2];
+      double w = kp * hz;
+      for (int jj = 0; jj < per[1]; ++jj) {
+        double v;
+        if (sy[t] == 1) {
+          int jp = g1lo[1] + (jj + jshift) % per[1];
+          v = jp * hy;
+        } else {
+          v = 2.0 * ymin + jj * hy;
+        }
+        for (int ii = 0; ii < per[0]; ++ii) {
+          double u;
+          if (sx[t] == 1) {
+            int ip = g1lo[0] + (ii + ishift) % per[0];
+            u = ip * hx;
+          } else {
+            u = 2.0 * xmin + ii * hx;
+          }
+          c[IDX3(ii, jj, kk, per[0], per[1])].re = oc_rfun(u, v, w, gamma, a, b, hz, sx[t], sy[t]);
+          c[IDX3(ii, jj, kk, per[0], per[1])].im = 0.0;
+        }
+      }
+    }
+    const int dir[3] = {sx[t], sy[t], 1};
+    oc_ccfft3d(g[t], per[0], per[1], per[2], dir);
+  }
+}
+
+/* ========================================================================== */
+/* Solvers                                                                     */
+/* ========================================================================== */
+
+/* osc_freespace_solver2 (open_spacecharge_mod.f90:535-597) */
+void oc_freespace_solver2(int nx, int ny, int nz, const double *rho, double gamma,
+                          const double delta[3], double *efield, double *phi,
+                          const double *offset) {
+  int nx2 = 2 * nx, ny2 = 2 * ny, nz2 = 2 * nz;
+  int64_t N2 = (int64_t)nx2 * ny2 * nz2, n = (int64_t)nx * ny * nz;
+  cplx *crho = calloc(N2, sizeof(cplx)), *cgrn = malloc(N2 * sizeof(cplx));
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) crho[IDX3(i, j, k, nx2, ny2)].re = rho[IDX3(i, j, k, nx, ny)];
+  const int fwd[3] = {1, 1, 1}, inv[3] = {-1, -1, -1};
+  oc_ccfft3d((double *)crho, nx2, ny2, nz2, fwd);
+  for (int icomp = 0; icomp <= 3; ++icomp) {
+    if (icomp == 0 && !phi) continue;
+    if (icomp == 1 && !efield) break;
+    oc_get_cgrn_freespace((double *)cgrn, nx2, ny2, nz2, delta, gamma, icomp, offset);
+    oc_ccfft3d((double *)cgrn, nx2, ny2, nz2, fwd);
+    for (int64_t i = 0; i < N2; ++i) cgrn[i] = cmul(crho[i], cgrn[i]);
+    oc_ccfft3d((double *)cgrn, nx2, ny2, nz2, inv);
+    int ish = nx - 1, jsh = ny - 1, ksh = nz - 1;
+    double factr = oc_fpei_solver2 / (double)(nx2 * ny2 * nz2);
+    double *out = icomp == 0 ? phi : efield + (icomp - 1) * n;
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+          out[IDX3(i, j, k, nx, ny)] = factr * cgrn[IDX3(i + ish, j + jsh, k + ksh, nx2, ny2)].re;
+  }
+  free(crho);
+  free(cgrn);
+}
+
+/* space_charge_3d (open_spacecharge_mod.f90:431-489) */
+void oc_space_charge_3d(const int nlo[3], const int nhi[3], const double mn[3],
+                        const double mx[3], const double delta[3], double gamma,
+                        const double *rho, const double *offset, int at_cathode,
+                        int calc_bfield, double *efield, double *bfield) {
+  int nx = nhi[0] - nlo[0] + 1, ny = nhi[1] - nlo[1] + 1, nz = nhi[2] - nlo[2] + 1;
+  int64_t n = (int64_t)nx * ny * nz;
+  double off[3] = {0.0, 0.0, 0.0};
+  if (offset) memcpy(off, offset, sizeof(off));
+  oc_freespace_solver2(nx, ny, nz, rho, gamma, delta, efield, NULL, off);
+  double beta = 0.0, c = oc_clight_sp;
+  if (calc_bfield) {
+    beta = sqrt(1.0 - 1.0 / (gamma * gamma));
+    for (int64_t i = 0; i < n; ++i) {
+      bfield[i] = -((beta / c) * efield[n + i]);
+      bfield[n + i] = (beta / c) * efield[i];
+      bfield[2 * n + i] = 0.0;
+    }
+  }
+  if (!at_cathode) return;
+  off[2] = off[2] + 2.0 * mn[2] + (mx[2] - mn[2]); /* (a + b) + c, as upstream */
+  double *rflip = malloc(sizeof(double) * n), *img = malloc(sizeof(double) * 3 * n);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        rflip[IDX3(i, j, k, nx, ny)] = -rho[IDX3(i, j, nz - 1 - k, nx, ny)];
+  oc_freespace_solver2(nx, ny, nz, rflip, gamma, delta, img, NULL, off);
+  if (calc_bfield)
+    for (int64_t i = 0; i < n; ++i) {
+      bfield[i] = bfield[i] + (beta / c) * img[n + i];
+      bfield[n + i] = bfield[n + i] - (beta / c) * img[i];
+    }
+  for (int64_t i = 0; i < 3 * n; ++i) efield[i] = efield[i] + img[i];
+  free(rflip);
+  free(img);
+}
+
+/* getrhotilde (core:435-449) */
+static cplx *getrhotilde(const double *rho, int nx, int ny, int nz, const int per[3]) {
+  cplx *c = calloc((size_t)per[0] * per[1] * per[2], sizeof(cplx));
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) c[IDX3(i, j, k, per[0], per[1])].re = rho[IDX3(i, j, k, nx, ny)];
+  const int fwd[3] = {1, 1, 1};
+  oc_ccfft3d((double *)c, per[0], per[1], per[2], fwd);
+  return c;
+}
+
+/* conv3d (core:503-529) / imageconvcorr3d (core:1237-1264) / one term of
+   fftconvcorr3d (core:763-818): ccon = crho*qgrn, inverse with signs dir */
+static cplx *convterm(const cplx *crho, const cplx *g, const int per[3], const int dir[3]) {
+  int64_t N = (int64_t)per[0] * per[1] * per[2];
+  cplx *c = malloc(N * sizeof(cplx));
+  for (int64_t i = 0; i < N; ++i) c[i] = cmul(crho[i], g[i]);
+  oc_ccfft3d((double *)c, per[0], per[1], per[2], dir);
+  return c;
+}
+
+static double conv_factr(const int per[3], double fpei) {
+  return 1.0 / ((1.0 * per[0]) * (1.0 * per[1]) * (1.0 * per[2])) * fpei * 1.0;
+}
+
+static void extract_first(const cplx *c, const int per[3], double factr, double *out, int nx,
+                          int ny, int nz) {
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        out[IDX3(i, j, k, nx, ny)] = factr * c[IDX3(i, j, k, per[0], per[1])].re;
+}
+
+static void conv3d(const cplx *crho, const cplx *g, const int per[3], double *out, int nx,
+                   int ny, int nz, const int dir[3], double fpei) {
+  cplx *c = convterm(crho, g, per, dir);
+  extract_first(c, per, conv_factr(per, fpei), out, nx, ny, nz);
+  free(c);
+}
+
+/* finite-difference gradient with one-sided x2 edges (core:87-104) */
+static void fd_gradient(const double *phi, int nx, int ny, int nz, const double delta[3],
+                        double gam0, double *ef) {
+  int64_t n = (int64_t)nx * ny * nz;
+  double dx = delta[0], dy = delta[1], dz = delta[2];
+  for (int k = 0; k < nz; ++k) {
+    int km1 = k - 1 < 0 ? k : k - 1, kp1 = k + 1 > nz - 1 ? k : k + 1;
+    double zfac = (km1 == k || kp1 == k) ? 2.0 : 1.0;
+    for (int j = 0; j < ny; ++j) {
+      int jm1 = j - 1 < 0 ? j : j - 1, jp1 = j + 1 > ny - 1 ? j : j + 1;
+      double yfac = (jm1 == j || jp1 == j) ? 2.0 : 1.0;
+      for (int i = 0; i < nx; ++i) {
+        int im1 = i - 1 < 0 ? i : i - 1, ip1 = i + 1 > nx - 1 ? i : i + 1;
+        double xfac = (im1 == i || ip1 == i) ? 2.0 : 1.0;
+        int64_t c = IDX3(i, j, k, nx, ny);
+        ef[c] = -((phi[IDX3(ip1, j, k, nx, ny)] - phi[IDX3(im1, j, k, nx, ny)]) / (2.0 * dx) * gam0 * xfac);
+        ef[n + c] = -((phi[IDX3(i, jp1, k, nx, ny)] - phi[IDX3(i, jm1, k, nx, ny)]) / (2.0 * dy) * gam0 * yfac);
+        ef[2 * n + c] = -((phi[IDX3(i, j, kp1, nx, ny)] - phi[IDX3(i, j, km1, nx, ny)]) / (2.0 * dz) / gam0 * zfac);
+      }
+    }
+  }
+}
+
+static void bfield_core(const double *ef, int64_t n, double gam0, double *bf) {
+  double beta0 = sqrt(1.0 - 1.0 / (gam0 * gam0)), c = oc_clight_dp;
+  for (int64_t i = 0; i < n; ++i) { /* core:127-136 */
+    bf[i] = -ef[n + i] * beta0 / c;
+    bf[n + i] = ef[i] * beta0 / c;
+    bf[2 * n + i] = 0.0;
+  }
+}
+
+/* osc_freespace_solver (core:30-138) */
+int oc_space_charge_freespace(const int nlo[3], const int nhi[3], const int npad[3],
+                              const double delta[3], double gamma, const double *rho,
+                              int direct, int igf, double *phi, double *efield,
+                              double *bfield) {
+  if (igf == 0 && direct == 1) return -1; /* :58-62 */
+  int nx = nhi[0] - nlo[0] + 1, ny = nhi[1] - nlo[1] + 1, nz = nhi[2] - nlo[2] + 1;
+  int64_t n = (int64_t)nx * ny * nz;
+  int lo[3], per[3];
+  legacy_bounds(nlo, nhi, npad, lo, per);
+  int64_t N = (int64_t)per[0] * per[1] * per[2];
+  cplx *g = malloc(N * sizeof(cplx));
+  const int inv[3] = {-1, -1, -1};
+  cplx *crho = getrhotilde(rho, nx, ny, nz, per);
+  if (direct == 0) {
+    oc_getgrnfree((double *)g, nlo, nhi, npad, delta, gamma, 0, igf);
+    conv3d(crho, g, per, phi, nx, ny, nz, inv, oc_fpei_conv3d);
+    fd_gradient(phi, nx, ny, nz, delta, gamma, efield);
+  } else {
+    for (int icomp = 1; icomp <= 3; ++icomp) {
+      oc_getgrnfree((double *)g, nlo, nhi, npad, delta, gamma, icomp, igf);
+      conv3d(crho, g, per, efield + (icomp - 1) * n, nx, ny, nz, inv, oc_fpei_conv3d);
+    }
+  }
+  bfield_core(efield, n, gamma, bfield);
+  free(crho);
+  free(g);
+  return 0;
+}
+
+/* osc_cathodeimages_solver (core:905-1083) */
+int oc_space_charge_cathodeimages(const int nlo[3], const int nhi[3], const int npad[3],
+                                  const double umin[3], const double delta[3], double gamma,
+                                  double *rho, int direct, int igf, int method, double *phi,
+                                  double *efield, double *bfield) {
+  if (igf == 0 && direct == 1) return -1;
+  int nx = nhi[0] - nlo[0] + 1, ny = nhi[1] - nlo[1] + 1, nz = nhi[2] - nlo[2] + 1;
+  int64_t n = (int64_t)nx * ny * nz;
+  int lo[3], per[3];
+  legacy_bounds(nlo, nhi, npad, lo, per);
+  int64_t N = (int64_t)per[0] * per[1] * per[2];
+  cplx *g = malloc(N * sizeof(cplx));
+  const int inv[3] = {-1, -1, -1}, invcorr[3] = {-1, -1, 1};
+  double *eimg = malloc(sizeof(double) * 3 * n);
+  cplx *crho = getrhotilde(rho, nx, ny, nz, per);
+  /* the z-flip of rho used by the shift method (core:1047-1057), in place */
+  int khalf = (nhi[2] - nlo[2] + 1) / 2;
+#define FLIPRHO()                                                                 \
+  for (int i = 0; i < nx; ++i)                                                    \
+    for (int j = 0; j < ny; ++j)                                                  \
+      for (int k = 0; k < khalf; ++k) {                                           \
+        double r1 = rho[IDX3(i, j, k, nx, ny)];                                   \
+        rho[IDX3(i, j, k, nx, ny)] = rho[IDX3(i, j, nz - 1 - k, nx, ny)];         \
+        rho[IDX3(i, j, nz - 1 - k, nx, ny)] = r1;                                 \
+      }
+  double umax3 = umin[2] + (nhi[2] - nlo[2]) * delta[2];
+  double zshft = umin[2] + umax3;
+  if (direct == 0) {
+    double *phiimg = malloc(sizeof(double) * n);
+    oc_getgrnfree((double *)g, nlo, nhi, npad, delta, gamma, 0, igf);
+    conv3d(crho, g, per, phi, nx, ny, nz, inv, oc_fpei_conv3d);
+    if (method == 1) {
+      oc_getgrnimageconvcorr((double *)g, nlo, nhi, npad, umin, delta, gamma, 0, igf);
+      conv3d(crho, g, per, phiimg, nx, ny, nz, invcorr, oc_fpei_conv3d);
+    } else {
+      FLIPRHO();
+      free(crho);
+      crho = getrhotilde(rho, nx, ny, nz, per);
+      oc_getgrnimageshift((double *)g, nlo, nhi, npad, delta, gamma, zshft, 0, igf);
+      conv3d(crho, g, per, phiimg, nx, ny, nz, inv, oc_fpei_conv3d);
+    }
+    for (int64_t i = 0; i < n; ++i) phi[i] = phi[i] - phiimg[i];
+    fd_gradient(phi, nx, ny, nz, delta, gamma, efield);
+    fd_gradient(phiimg, nx, ny, nz, delta, gamma, eimg);
+    free(phiimg);
+  } else {
+    for (int icomp = 1; icomp <= 3; ++icomp) {
+      oc_getgrnfree((double *)g, nlo, nhi, npad, delta, gamma, icomp, igf);
+      conv3d(crho, g, per, efield + (icomp - 1) * n, nx, ny, nz, inv, oc_fpei_conv3d);
+    }
+    if (method == 1) {
+      for (int icomp = 1; icomp <= 3; ++icomp) {
+        oc_getgrnimageconvcorr((double *)g, nlo, nhi, npad, umin, delta, gamma, icomp, igf);
+        conv3d(crho, g, per, eimg + (icomp - 1) * n, nx, ny, nz, invcorr, oc_fpei_conv3d);
+        for (int64_t i = 0; i < n; ++i)
+          efield[(icomp - 1) * n + i] = efield[(icomp - 1) * n + i] - eimg[(icomp - 1) * n + i];
+      }
+    } else {
+      FLIPRHO();
+      free(crho);
+      crho = getrhotilde(rho, nx, ny, nz, per);
+      for (int icomp = 1; icomp <= 3; ++icomp) {
+        oc_getgrnimageshift((double *)g, nlo, nhi, npad, delta, gamma, zshft, icomp, igf);
+        conv3d(crho, g, per, eimg + (icomp - 1) * n, nx, ny, nz, inv, oc_fpei_conv3d);
+        for (int64_t i = 0; i < n; ++i)
+          efield[(icomp - 1) * n + i] = efield[(icomp - 1) * n + i] - eimg[(icomp - 1) * n + i];
+      }
+    }
+  }
+#undef FLIPRHO
+  double beta0 = sqrt(1.0 - 1.0 / (gamma * gamma)), c = oc_clight_dp;
+  for (int64_t i = 0; i < n; ++i) { /* :1069-1081 */
+    bfield[i] = -(efield[n + i] + 2.0 * eimg[n + i]) * beta0 / c;
+    bfield[n + i] = (efield[i] + 2.0 * eimg[i]) * beta0 / c;
+    bfield[2 * n + i] = 0.0;
+  }
+  free(eimg);
+  free(crho);
+  free(g);
+  return 0;
+}
+
+/* osc_rectpipe_solver (core:536-624) + fftconvcorr3d (core:763-818) */
+int oc_space_charge_rectpipe(const int nlo[3], const int nhi[3], const int npad[3],
+                             const double delta[3], double gamma, const double *rho,
+                             double *const g[4], int direct, double *phi, double *efield,
+                             double *bfield) {
+  int nx = nhi[0] - nlo[0] + 1, ny = nhi[1] - nlo[1] + 1, nz = nhi[2] - nlo[2] + 1;
+  int64_t n = (int64_t)nx * ny * nz;
+  int lo[3], per[3];
+  legacy_bounds(nlo, nhi, npad, lo, per);
+  int64_t N = (int64_t)per[0] * per[1] * per[2];
+  int rc = 0;
+  if (direct == 0) {
+    cplx *crho = getrhotilde(rho, nx, ny, nz, per);
+    const int d1[3] = {-1, -1, -1}, d2[3] = {-1, 1, -1}, d3[3] = {1, -1, -1}, d4[3] = {1, 1, -1};
+    cplx *ccon = convterm(crho, (const cplx *)g[0], per, d1);
+    cplx *t = convterm(crho, (const cplx *)g[1], per, d2);
+    for (int64_t i = 0; i < N; ++i) {
+      ccon[i].re = ccon[i].re - t[i].re;
+      ccon[i].im = ccon[i].im - t[i].im;
+    }
+    free(t);
+    t = convterm(crho, (const cplx *)g[2], per, d3);
+    for (int64_t i = 0; i < N; ++i) {
+      ccon[i].re = ccon[i].re - t[i].re;
+      ccon[i].im = ccon[i].im - t[i].im;
+    }
+    free(t);
+    t = convterm(crho, (const cplx *)g[3], per, d4);
+    for (int64_t i = 0; i < N; ++i) {
+      ccon[i].re = ccon[i].re + t[i].re;
+      ccon[i].im = ccon[i].im + t[i].im;
+    }
+    free(t);
+    extract_first(ccon, per, conv_factr(per, oc_fpei_pipe), phi, nx, ny, nz);
+    free(ccon);
+    free(crho);
+    fd_gradient(phi, nx, ny, nz, delta, gamma, efield);
+  } else {
+    rc = -1; /* upstream prints an error and leaves E unset (:608-610) */
+  }
+  bfield_core(efield, n, gamma, bfield);
+  return rc;
+}
+
+/* ========================================================================== */
+/* Gather                                                                      */
+/* ========================================================================== */
+
+/* interpolate_field (open_spacecharge_mod.f90:311-390) */
+void oc_interpolate_field(const int nlo[3], const int nhi[3], const double mn[3],
+                          const double delta[3], const double *efield, const double *bfield,
+                          int64_t n, const double *x, const double *y, const double *z,
+                          double *E, double *B, int8_t *clamped) {
+  int nx = nhi[0] - nlo[0] + 1, ny = nhi[1] - nlo[1] + 1, nz = nhi[2] - nlo[2] + 1;
+  int64_t nt = (int64_t)nx * ny * nz;
+  double hxi = 1.0 / delta[0], hyi = 1.0 / delta[1], hzi = 1.0 / delta[2];
+  for (int64_t p = 0; p < n; ++p) {
+    int ip = (int)floor((x[p] - mn[0]) * hxi + 1.0);
+    int jp = (int)floor((y[p] - mn[1]) * hyi + 1.0);
+    int kp = (int)floor((z[p] - mn[2]) * hzi + 1.0);
+    int fl = 0;
+    if (ip < 1 || ip > nhi[0] - 1) {
+      fl = 1;
+      ip = ip < 1 ? 1 : nhi[0] - 1;
+    }
+    if (jp < 1 || jp > nhi[1] - 1) {
+      fl = 1;
+      jp = jp < 1 ? 1 : nhi[1] - 1;
+    }
+    if (kp < 1 || kp > nhi[2] - 1) {
+      fl = 1;
+      kp = kp < 1 ? 1 : nhi[2] - 1;
+    }
+    if (clamped) clamped[p] = (int8_t)fl;
+    double ab = ((mn[0] - x[p]) + ip * delta[0]) * hxi;
+    double de = ((mn[1] - y[p]) + jp * delta[1]) * hyi;
+    double gh = ((mn[2] - z[p]) + kp * delta[2]) * hzi;
+    int i0 = ip - 1, j0 = jp - 1, k0 = kp - 1, i1 = ip, j1 = jp, k1 = kp;
+    for (int f = 0; f < 2; ++f) {
+      const double *F = f == 0 ? efield : bfield;
+      double *O = f == 0 ? E : B;
+      if (!F || !O) continue;
+      for (int c = 0; c < 3; ++c) {
+        const double *G = F + c * nt;
+#define G3(i, j, k) G[IDX3(i, j, k, nx, ny)]
+        O[3 * p + c] = G3(i0, j0, k0) * ab * de * gh + G3(i0, j1, k0) * ab * (1. - de) * gh +
+                       G3(i0, j1, k1) * ab * (1. - de) * (1. - gh) +
+                       G3(i0, j0, k1) * ab * de * (1. - gh) +
+                       G3(i1, j0, k1) * (1. - ab) * de * (1. - gh) +
+                       G3(i1, j1, k1) * (1. - ab) * (1. - de) * (1. - gh) +
+                       G3(i1, j1, k0) * (1. - ab) * (1. - de) * gh +
+                       G3(i1, j0, k0) * (1. - ab) * de * gh;
+#undef G3
+      }
+    }
+  }
+}
+
+/* ========================================================================== */
+/* Pipe Green file: Fortran unformatted sequential, 4-byte record markers      */
+/* ========================================================================== */
+
+/* osc_write_rectpipe_grn (core:851-870) */
+int oc_write_rectpipe_grn(const char *path, double a, double b, const double delta[3],
+                          const double umin[3], const double umax[3], const int nlo[3],
+                          const int nhi[3], const int period[3], double gamma,
+                          double *const g[4]) {
+  FILE *f = fopen(path, "wb");
+  if (!f) return -1;
+  int32_t rec1 = 11 * 8 + 9 * 4 + 8;
+  double r[11] = {a, b, delta[0], delta[1], delta[2], umin[0], umin[1], umin[2],
+                  umax[0], umax[1], umax[2]};
+  int32_t iv[9] = {nlo[0], nlo[1], nlo[2], nhi[0], nhi[1], nhi[2], period[0], period[1], period[2]};
+  fwrite(&rec1, 4, 1, f);
+  fwrite(r, 8, 11, f);
+  fwrite(iv, 4, 9, f);
+  fwrite(&gamma, 8, 1, f);
+  fwrite(&rec1, 4, 1, f);
+  int64_t each = (int64_t)period[0] * period[1] * period[2] * 16;
+  int32_t rec2 = (int32_t)(4 * each);
+  fwrite(&rec2, 4, 1, f);
+  for (int t = 0; t < 4; ++t) fwrite(g[t], 1, each, f);
+  fwrite(&rec2, 4, 1, f);
+  return fclose(f) == 0 ? 0 : -1;
+}
+
+/* osc_read_rectpipe_grn (core:825-844); hdr_real = a,b,dx,dy,dz,xmin..zmax,gamma */
+int oc_read_rectpipe_grn(const char *path, double hdr_real[12], int hdr_int[9], double *g[4],
+                         int64_t n_each) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return -1;
+  int32_t m1, m2;
+  int ok = fread(&m1, 4, 1, f) == 1 && m1 == 132;
+  ok = ok && fread(hdr_real, 8, 11, f) == 11 && fread(hdr_int, 4, 9, f) == 9 &&
+       fread(hdr_real + 11, 8, 1, f) == 1 && fread(&m2, 4, 1, f) == 1 && m2 == 132;
+  ok = ok && fread(&m1, 4, 1, f) == 1;
+  for (int t = 0; ok && t < 4; ++t) ok = fread(g[t], 16, n_each, f) == (size_t)n_each;
+  fclose(f);
+  return ok ? 0 : -1;
+}
