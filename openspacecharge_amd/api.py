@@ -140,13 +140,19 @@ class Mesh3D:
         """(Re)allocates rho, phi, efield, bfield when absent or of the wrong
         shape (open_spacecharge_mod.f90:193-219)."""
         sh = self.shape
+        if like is None and self.device == "cuda":
+            import torch
+            like = torch.empty(1, dtype=torch.float64, device="cuda")
         if self.rho is None or tuple(self.rho.shape) != sh:
-            if like is None and self.device == "cuda":
-                import torch
-                like = torch.empty(1, dtype=torch.float64, device="cuda")
+            self.rho = self.phi = self.efield = self.bfield = None
+        if self.rho is None:
             self.rho = _zeros_like_layout(like, sh)
+        like = self.rho
+        if self.phi is None or tuple(self.phi.shape) != sh:
             self.phi = _zeros_like_layout(like, sh)
+        if self.efield is None or tuple(self.efield.shape) != sh + (3,):
             self.efield = _zeros_like_layout(like, sh + (3,))
+        if self.bfield is None or tuple(self.bfield.shape) != sh + (3,):
             self.bfield = _zeros_like_layout(like, sh + (3,))
 
 

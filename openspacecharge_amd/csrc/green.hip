@@ -40,7 +40,7 @@ __global__ void k_prim_offset(double* __restrict__ F, int L0, int L1, int L2, do
 // 8-corner difference over 1:size-1 with the last planes left raw
 // (open_spacecharge_mod.f90:697-705); output z index circularly shifted by sh
 __global__ void k_diff_offset(const double* __restrict__ F, double* __restrict__ G, int L0, int L1,
-                              int L2, int sh) {
+                              int L2, int sh, int zero_raw) {
   const int64_t N = int64_t(L0) * L1 * L2;
   const int64_t P = int64_t(L0) * L1;
   for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < N;
@@ -56,7 +56,9 @@ __global__ void k_diff_offset(const double* __restrict__ F, double* __restrict__
       g = F[b + 1 + L0 + P] - F[b + L0 + P] - F[b + 1 + P] - F[b + 1 + L0] - F[b] + F[b + P] +
           F[b + L0] + F[b + 1];
     } else {
-      g = F[b];
+      // upstream keeps the raw primitive here; it never reaches the
+      // extracted octant, and zeroing it removes its round-off from the FFT
+      g = zero_raw ? 0.0 : F[b];
     }
     G[idx] = g;
   }
@@ -150,7 +152,7 @@ void build_green_real(osc_ctx* c, const GreenSpec& s, double* g) {
                        s.L[0], s.L[1], s.L[2], dx, dy, dz, umin, vmin, wmin, factor, s.icomp);
     OSC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_diff_offset, dim3(nblk(N)), dim3(kThreads), 0, c->stream,
-                       F.as<const double>(), g, s.L[0], s.L[1], s.L[2], s.out_shift_z);
+                       F.as<const double>(), g, s.L[0], s.L[1], s.L[2], s.out_shift_z, s.zero_raw);
     OSC_HIP(hipGetLastError());
   } else if (s.kind == GK_PIPE) {
     hipLaunchKernelGGL(k_green_pipe, dim3(nblk(N)), dim3(kThreads), 0, c->stream, g, s);

@@ -39,7 +39,7 @@ struct GreenSpec {
   int32_t shift[3];   // iperiod/2-(ipad+1)/2 (legacy / pipe)
   int32_t nlo3;
   int32_t out_shift_z;  // circular shift applied to the output z index
-  int32_t pad_;
+  int32_t zero_raw;     // offset kind: zero the raw last planes (never extracted)
   double delta[3], gamma, offset[3], zshft, umin[3], a, b;
 };
 GreenSpec make_green_spec();

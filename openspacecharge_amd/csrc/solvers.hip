@@ -49,6 +49,7 @@ GreenSpec offset_spec(const int L[3], const double delta[3], double gamma, int i
   }
   s.gamma = gamma;
   s.out_shift_z = shz;
+  s.zero_raw = 1;
   return s;
 }
 

@@ -45,6 +45,14 @@ def nsize(nlo, nhi):
     return tuple(int(h) - int(l) + 1 for l, h in zip(nlo, nhi))
 
 
+def _relcomp(a, b):
+    """max over trailing components of max|a-b| / max|b| (whole array if 1-D/3-D)."""
+    a, b = np.asarray(a), np.asarray(b)
+    if a.ndim == 4:
+        return max(_relcomp(a[..., c], b[..., c]) for c in range(a.shape[-1]))
+    return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
+
+
 class Oracle:
     """The C restatement (oracle/osc_oracle.c)."""
 
@@ -66,6 +74,31 @@ class Oracle:
         L.oc_igf.argtypes = [C.c_int, C.c_int] + [C.c_double] * 7
         L.oc_rfun.restype = C.c_double
         L.oc_rfun.argtypes = [C.c_double] * 7 + [C.c_int, C.c_int]
+        L.oc_set_perturb.argtypes = [C.c_uint64]
+        L.oc_set_zero_unused_planes.argtypes = [C.c_int]
+
+    # -- test knobs ----------------------------------------------------------
+    def set_perturb(self, seed: int) -> None:
+        """Perturb every IGF primitive result by -1/0/+1 ulp (seed 0 = off)."""
+        self.lib.oc_set_perturb(C.c_uint64(seed))
+
+    def set_zero_unused_planes(self, on: bool) -> None:
+        self.lib.oc_set_zero_unused_planes(int(bool(on)))
+
+    def floor(self, fn, seeds=(12345, 67890)):
+        """Conditioning floor of fn(): max spread of its (array) result when
+        the IGF primitives are perturbed by 1 ulp, over a few seeds.  Returns
+        (unperturbed result, floor as a fraction of max|result| per component)."""
+        base = fn()
+        worst = 0.0
+        for s in seeds:
+            self.set_perturb(s)
+            try:
+                pert = fn()
+            finally:
+                self.set_perturb(0)
+            worst = max(worst, _relcomp(pert, base))
+        return base, worst
 
     # -- FFT ---------------------------------------------------------------
     def ccfft3d(self, a, idir):

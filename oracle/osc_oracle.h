@@ -26,6 +26,12 @@ extern const double oc_fpei_pipe;      /* open_spacecharge_core_mod.f90:783 */
 extern const double oc_clight_sp;      /* open_spacecharge_mod.f90:436,546 */
 extern const double oc_clight_dp;      /* open_spacecharge_core_mod.f90:51,555,929 */
 
+/* ---- test knobs (defaults restate upstream exactly) ------------------------ */
+/* perturb every IGF primitive result by a pseudo-random -1/0/+1 ulp (0 = off) */
+void oc_set_perturb(uint64_t seed);
+/* zero the raw, never-used last planes of the offset-grid Green function */
+void oc_set_zero_unused_planes(int on);
+
 /* ---- 3-D complex FFT (fft_interface_mod.f90:96-143) ---------------------- */
 /* in place, unnormalised, axis d uses exp(idir[d] * 2*pi*i*j*k/n).           */
 /* returns 0, or -1 if a length has a prime factor > 23 (upstream stops).    */
