@@ -82,6 +82,17 @@ int osc_ctx_clear_caches(osc_ctx* ctx);
 int osc_ctx_synchronize(osc_ctx* ctx);
 const char* osc_ctx_last_error(const osc_ctx* ctx);
 int osc_abi_version(void);
+
+/* Per-stage timing with HIP events on the context stream (no counterpart
+ * upstream, which only times the whole solve with cpu_time,
+ * test_opensc.f90:175,219-220).  While enabled, every kernel launch is
+ * bracketed by events of its stage; osc_ctx_profile_read synchronises, adds
+ * the elapsed times (ms) and launch counts of each stage into the arrays
+ * (OSC_NSTAGE entries) and clears the records. */
+#define OSC_NSTAGE 16
+int osc_ctx_set_profiling(osc_ctx* ctx, int on);
+int osc_ctx_profile_read(osc_ctx* ctx, double* ms, int64_t* launches);
+const char* osc_stage_name(int stage);
 /* 1 if this transform length is supported (prime factors <= 23 and fits LDS) */
 int osc_fft_length_supported(int n);
 
@@ -97,6 +108,20 @@ int osc_fft_length_supported(int n);
 int osc_deposit_particles(osc_ctx* ctx, int64_t n, const double* xa, const double* ya,
                           const double* za, const double* qa, const double* total_charge,
                           int resize_mesh, osc_mesh* mesh, double* rho, int64_t* n_dropped);
+
+/* One tracking step: deposit -> space-charge solve -> gather at the same
+ * particles.  No single upstream routine: it replaces the caller's sequence
+ * deposit_particles (open_spacecharge_mod.f90:156-288), space_charge_3d
+ * (:431-489) and interpolate_field per particle (:311-390), and gives the
+ * same results as osc_deposit_particles + osc_space_charge_3d +
+ * osc_interpolate_field.  The gather reuses the deposit's cell-sorted
+ * particle order.  E and B are (3,n); B requires calc_bfield; mesh arrays
+ * rho (nx,ny,nz), efield and bfield (nx,ny,nz,3) are outputs. */
+int osc_space_charge_step(osc_ctx* ctx, int64_t n, const double* xa, const double* ya,
+                          const double* za, const double* qa, const double* total_charge,
+                          int resize_mesh, const double* offset, int at_cathode, int calc_bfield,
+                          osc_mesh* mesh, double* rho, double* efield, double* bfield, double* E,
+                          double* B, int64_t* n_dropped, int64_t* n_clamped);
 
 /* get_mesh_quantities (code/test_mod.f90:288-358): harness mesh rule with one
  * guard node per side; isetzmin=1 puts zmin at 1e-9 (cathode runs). */

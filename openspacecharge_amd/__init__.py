@@ -12,6 +12,6 @@ from .api import (Mesh3D, OpenSCStop, ccfft3d, context, deposit_particles,  # no
                   osc_getgrnimageconvcorr, osc_getgrnimageshift, osc_getgrnpipe,
                   osc_read_rectpipe_grn, osc_write_rectpipe_grn, print_mesh3d, set_context,
                   space_charge_3d, space_charge_cathodeimages, space_charge_freespace,
-                  space_charge_rectpipe)
+                  space_charge_rectpipe, space_charge_step)
 
 __version__ = "0.1.0"

@@ -74,8 +74,14 @@ SIGNATURES = {
     "osc_ctx_clear_caches": (C.c_int, [_ctxp]),
     "osc_ctx_synchronize": (C.c_int, [_ctxp]),
     "osc_ctx_last_error": (C.c_char_p, [_ctxp]),
+    "osc_ctx_set_profiling": (C.c_int, [_ctxp, C.c_int]),
+    "osc_ctx_profile_read": (C.c_int, [_ctxp, _dp, C.POINTER(_i64)]),
+    "osc_stage_name": (C.c_char_p, [C.c_int]),
     "osc_deposit_particles": (C.c_int, [_ctxp, _i64, _vp, _vp, _vp, _vp, _dp, C.c_int, _meshp,
                                         _vp, C.POINTER(_i64)]),
+    "osc_space_charge_step": (C.c_int, [_ctxp, _i64, _vp, _vp, _vp, _vp, _dp, C.c_int, _dp, C.c_int,
+                                        C.c_int, _meshp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i64),
+                                        C.POINTER(_i64)]),
     "osc_get_mesh_quantities": (C.c_int, [_ctxp, _i64, _vp, _vp, _vp, _i32p, _i32p, C.c_int, _dp,
                                           _dp]),
     "osc_depose_rho_scalar": (C.c_int, [_ctxp, _i64, _vp, _vp, _vp, _i32p, _i32p, C.c_double, _dp,
@@ -177,6 +183,19 @@ class Context:
 
     def synchronize(self):
         self.check(self.lib.osc_ctx_synchronize(self.h))
+
+    NSTAGE = 16
+
+    def set_profiling(self, on: bool):
+        self.check(self.lib.osc_ctx_set_profiling(self.h, int(bool(on))))
+
+    def profile_read(self) -> dict:
+        """{stage name: (total ms, launches)} since the last read (synchronises)."""
+        ms = (C.c_double * self.NSTAGE)()
+        n = (C.c_int64 * self.NSTAGE)()
+        self.check(self.lib.osc_ctx_profile_read(self.h, ms, n))
+        return {self.lib.osc_stage_name(i).decode(): (ms[i], n[i]) for i in range(self.NSTAGE)
+                if n[i] > 0}
 
     def close(self):
         if getattr(self, "h", None):

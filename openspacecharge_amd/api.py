@@ -201,6 +201,47 @@ def deposit_particles(xa, ya, za, mesh3d: Mesh3D, qa=None, total_charge=None, re
     return int(nd.value)
 
 
+def space_charge_step(xa, ya, za, mesh3d: Mesh3D, qa=None, total_charge=None, resize_mesh=True,
+                      offset=None, at_cathode=False, calc_bfield=False, want_B=None,
+                      ctx: Optional[K.Context] = None):
+    """One tracking step at the same particles: deposit_particles ->
+    space_charge_3d -> interpolate_field for every particle, in one call that
+    reuses the deposit's cell-sorted order for the gather.  Results equal the
+    three separate calls.  Returns E (n,3) (and B (n,3) when calc_bfield)."""
+    ctx = ctx or context()
+    kx, px = _vec(xa)
+    ky, py = _vec(ya)
+    kz, pz = _vec(za)
+    kq, pq = _vec(qa)
+    n = int(kx.shape[0])
+    want_B = calc_bfield if want_B is None else want_B
+    mesh3d.allocate(like=kx if _is_torch(kx) else None)
+    dev = _is_torch(kx)
+    if dev:
+        import torch
+        E = torch.empty((n, 3), dtype=torch.float64, device=kx.device)
+        B = torch.empty((n, 3), dtype=torch.float64, device=kx.device) if want_B else None
+        pE, pB = E.data_ptr(), (B.data_ptr() if B is not None else None)
+    else:
+        E = np.empty((n, 3))
+        B = np.empty((n, 3)) if want_B else None
+        pE, pB = E.ctypes.data, (B.ctypes.data if B is not None else None)
+    m = mesh3d.to_c()
+    tc = C.c_double(float(total_charge)) if total_charge is not None else None
+    nt = mesh3d.n
+    rc = ctx.lib.osc_space_charge_step(
+        ctx.h, n, px, py, pz, pq, C.byref(tc) if tc is not None else None, int(bool(resize_mesh)),
+        K.d3(offset) if offset is not None else None, int(bool(at_cathode)), int(bool(calc_bfield)),
+        C.byref(m), _ptr(mesh3d.rho, nt, True), _ptr(mesh3d.efield, 3 * nt, True),
+        _ptr(mesh3d.bfield, 3 * nt, True) if calc_bfield else None, pE, pB, None, None)
+    mesh3d.from_c(m)
+    if rc == K.OSC_ERR_NO_CHARGE:
+        print(" Error, must specify qa or total_charge")
+        return None
+    ctx.check(rc)
+    return (E, B) if want_B else E
+
+
 def get_mesh_quantities(xa, ya, za, nlo, nhi, isetzmin=0, ctx: Optional[K.Context] = None):
     """get_mesh_quantities (code/test_mod.f90:288-358): (delta, umin)."""
     ctx = ctx or context()

@@ -99,6 +99,11 @@ OSC_API int osc_ctx_destroy(osc_ctx* c) {
       c->pipe.spec32[t].reset();
     }
     c->pool.release_all();
+    for (auto& r : c->prof_recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : c->prof_free) (void)hipEventDestroy(e);
     for (auto& kv : c->tw64) (void)hipFree(kv.second);
     for (auto& kv : c->tw32) (void)hipFree(kv.second);
     if (c->counters) (void)hipFree(c->counters);
@@ -159,9 +164,91 @@ OSC_API const char* osc_ctx_last_error(const osc_ctx* c) {
   return c ? c->last_error.c_str() : "null context";
 }
 
+static_assert(OSC_NSTAGE == ST_COUNT, "stage table");
+
+OSC_API const char* osc_stage_name(int s) {
+  static const char* names[ST_COUNT] = {
+      "deposit.bbox", "deposit.keys", "deposit.sort", "deposit.cells", "deposit.nodes",
+      "green.eval",   "green.fft",    "fft.fwd_x",    "fft.fwd_y",     "fft.fwd_z",
+      "fft.inv_zmul", "fft.inv_y",    "fft.inv_x",    "gather",        "fd_bfield",
+      "other"};
+  return (s >= 0 && s < ST_COUNT) ? names[s] : "?";
+}
+
+OSC_API int osc_ctx_set_profiling(osc_ctx* c, int on) {
+  return guarded(c, [&] {
+    c->prof = on != 0;
+    return OSC_OK;
+  });
+}
+
+OSC_API int osc_ctx_profile_read(osc_ctx* c, double* ms, int64_t* launches) {
+  return guarded(c, [&] {
+    OSC_HIP(hipStreamSynchronize(c->stream));
+    for (auto& r : c->prof_recs) {
+      float t = 0.f;
+      OSC_HIP(hipEventElapsedTime(&t, r.a, r.b));
+      c->prof_ms[r.stage] += t;
+      c->prof_n[r.stage] += 1;
+      c->prof_free.push_back(r.a);
+      c->prof_free.push_back(r.b);
+    }
+    c->prof_recs.clear();
+    for (int s = 0; s < ST_COUNT; ++s) {
+      if (ms) ms[s] = c->prof_ms[s];
+      if (launches) launches[s] = c->prof_n[s];
+      c->prof_ms[s] = 0;
+      c->prof_n[s] = 0;
+    }
+    return OSC_OK;
+  });
+}
+
 // ---------------------------------------------------------------------------
 // deposition
 // ---------------------------------------------------------------------------
+// deposit_particles body on staged (device) coordinates; rho is device memory
+static int deposit_staged(osc_ctx* c, Staging& st, int64_t n, const double* x, const double* y,
+                          const double* z, const double* qa, const double* total_charge,
+                          int resize_mesh, osc_mesh* m, double* rho, SortedParticles* keep) {
+  int nn[3];
+  mesh_dims(m, nn);
+  if (resize_mesh) {  // :176-189
+    double lo[3], hi[3], d[3];
+    bbox(c, n, x, y, z, lo, hi);
+    for (int a = 0; a < 3; ++a) {
+      d[a] = (hi[a] - lo[a]) / double(m->nhi[a] - m->nlo[a]);
+      lo[a] = lo[a] - 1.0e-6 * d[a];
+      hi[a] = hi[a] + 1.0e-6 * d[a];
+      d[a] = (hi[a] - lo[a]) / double(m->nhi[a] - m->nlo[a]);
+      m->min[a] = lo[a];
+      m->max[a] = hi[a];
+      m->delta[a] = d[a];
+    }
+  }
+  DepositArgs a{};
+  a.n = n;
+  a.x = x;
+  a.y = y;
+  a.z = z;
+  if (qa) {  // :244-245
+    a.q = st.in(qa, size_t(n));
+    m->charge = n > 0 ? device_sum(c, a.q, n) : 0.0;
+  } else if (total_charge) {  // :246-248
+    m->charge = *total_charge;
+    a.q_each = *total_charge / double(n);
+  } else {
+    return fail(c, OSC_ERR_NO_CHARGE, "Error, must specify qa or total_charge");
+  }
+  for (int d = 0; d < 3; ++d) {
+    a.nlo[d] = m->nlo[d];
+    a.nn[d] = nn[d];
+    a.mn[d] = m->min[d];
+    a.delta[d] = m->delta[d];
+  }
+  deposit_cic(c, a, rho, keep);
+  return OSC_OK;
+}
 OSC_API int osc_deposit_particles(osc_ctx* c, int64_t n, const double* xa, const double* ya,
                                   const double* za, const double* qa, const double* total_charge,
                                   int resize_mesh, osc_mesh* m, double* rho, int64_t* n_dropped) {
@@ -176,43 +263,60 @@ OSC_API int osc_deposit_particles(osc_ctx* c, int64_t n, const double* xa, const
     const double* x = st.in(xa, size_t(n));
     const double* y = st.in(ya, size_t(n));
     const double* z = st.in(za, size_t(n));
-    if (resize_mesh) {  // :176-189
-      double lo[3], hi[3], d[3];
-      bbox(c, n, x, y, z, lo, hi);
-      for (int a = 0; a < 3; ++a) {
-        d[a] = (hi[a] - lo[a]) / double(m->nhi[a] - m->nlo[a]);
-        lo[a] = lo[a] - 1.0e-6 * d[a];
-        hi[a] = hi[a] + 1.0e-6 * d[a];
-        d[a] = (hi[a] - lo[a]) / double(m->nhi[a] - m->nlo[a]);
-        m->min[a] = lo[a];
-        m->max[a] = hi[a];
-        m->delta[a] = d[a];
-      }
-    }
-    DepositArgs a{};
-    a.n = n;
-    a.x = x;
-    a.y = y;
-    a.z = z;
-    if (qa) {  // :244-245
-      a.q = st.in(qa, size_t(n));
-      m->charge = n > 0 ? device_sum(c, a.q, n) : 0.0;
-    } else if (total_charge) {  // :246-248
-      m->charge = *total_charge;
-      a.q_each = *total_charge / double(n);
-    } else {
-      st.finish();
+    // upstream resizes the mesh, then returns without touching rho (:249-252)
+    double* r = (qa || total_charge) ? st.out(rho, size_t(prod3(nn))) : nullptr;
+    const int rc = deposit_staged(c, st, n, x, y, z, qa, total_charge, resize_mesh, m, r, nullptr);
+    if (rc == OSC_OK && n_dropped) *n_dropped = read_counter(c, 0);
+    st.finish();
+    return rc;
+  });
+}
+
+OSC_API int osc_space_charge_step(osc_ctx* c, int64_t n, const double* xa, const double* ya,
+                                  const double* za, const double* qa, const double* total_charge,
+                                  int resize_mesh, const double* offset, int at_cathode,
+                                  int calc_bfield, osc_mesh* m, double* rho, double* efield,
+                                  double* bfield, double* E, double* B, int64_t* n_dropped,
+                                  int64_t* n_clamped) {
+  return guarded(c, [&]() -> int {
+    if (!m || !rho || !efield || n < 1 || !xa || !ya || !za || (calc_bfield && !bfield) ||
+        (B && !calc_bfield))
+      return fail(c, OSC_ERR_ARG, "space_charge_step: bad arguments");
+    if (!qa && !total_charge)
       return fail(c, OSC_ERR_NO_CHARGE, "Error, must specify qa or total_charge");
+    int nn[3], L[3];
+    mesh_dims(m, nn);
+    if (nn[0] < 2 || nn[1] < 2 || nn[2] < 2)
+      return fail(c, OSC_ERR_ARG, "space_charge_step: mesh needs >= 2 nodes per axis");
+    for (int a = 0; a < 3; ++a) L[a] = 2 * nn[a];
+    if (int rc = check_len(c, L)) return rc;
+    const int64_t nt = prod3(nn);
+    Staging st(c);
+    const double* x = st.in(xa, size_t(n));
+    const double* y = st.in(ya, size_t(n));
+    const double* z = st.in(za, size_t(n));
+    double* r = st.out(rho, size_t(nt));
+    double* e = st.out(efield, size_t(3 * nt));
+    double* b = calc_bfield ? st.out(bfield, size_t(3 * nt)) : nullptr;
+    double* Eo = st.out(E, size_t(3 * n));
+    double* Bo = st.out(B, size_t(3 * n));
+    SortedParticles sp;
+    int rc = deposit_staged(c, st, n, x, y, z, qa, total_charge, resize_mesh, m, r, &sp);
+    if (rc != OSC_OK) return rc;
+    space_charge_3d(c, *m, r, offset, at_cathode, calc_bfield, e, b);
+    if (Eo || Bo) {
+      InterpArgs ia{};
+      for (int d = 0; d < 3; ++d) {
+        ia.nlo[d] = m->nlo[d];
+        ia.nhi[d] = m->nhi[d];
+        ia.nn[d] = nn[d];
+        ia.mn[d] = m->min[d];
+        ia.delta[d] = m->delta[d];
+      }
+      interpolate_sorted(c, ia, e, b, sp, Eo, Bo, nullptr);
     }
-    for (int d = 0; d < 3; ++d) {
-      a.nlo[d] = m->nlo[d];
-      a.nn[d] = nn[d];
-      a.mn[d] = m->min[d];
-      a.delta[d] = m->delta[d];
-    }
-    double* r = st.out(rho, size_t(prod3(nn)));
-    const int64_t nd = deposit_cic(c, a, r);
-    if (n_dropped) *n_dropped = nd;
+    if (n_dropped) *n_dropped = read_counter(c, 0);
+    if (n_clamped) *n_clamped = read_counter(c, 2);
     st.finish();
     return OSC_OK;
   });
@@ -278,8 +382,8 @@ OSC_API int osc_depose_rho_scalar(osc_ctx* c, int64_t n, const double* xa, const
       a.delta[d] = delta[d];
     }
     double* r = st.out(rho, size_t(prod3(nn)));
-    const int64_t nd = deposit_cic(c, a, r);
-    if (ifail) *ifail = nd;
+    deposit_cic(c, a, r);
+    if (ifail) *ifail = read_counter(c, 0);
     st.finish();
     return OSC_OK;
   });

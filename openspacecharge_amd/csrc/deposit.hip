@@ -157,10 +157,16 @@ __global__ void __launch_bounds__(RS_T)
   for (int d = threadIdx.x; d <= int(mask); d += RS_T) hist[int64_t(d) * nb + blockIdx.x] = h[d];
 }
 
+// payload gathered by the last pass: coordinates (and charges) in sorted order
+struct SortPayload {
+  const double *x, *y, *z, *q;
+  double *xs, *ys, *zs, *qs;
+};
+
 __global__ void __launch_bounds__(RS_T)
     k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                  uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n, int shift,
-                 int bits, const uint32_t* __restrict__ offs, int nb) {
+                 int bits, const uint32_t* __restrict__ offs, int nb, SortPayload pl) {
   __shared__ uint32_t run[256];
   __shared__ uint32_t wc[RS_W][256];
   const uint32_t mask = (1u << bits) - 1u;
@@ -191,6 +197,12 @@ __global__ void __launch_bounds__(RS_T)
       for (int w = 0; w < wave; ++w) pos += wc[w][d];
       kout[pos] = k;
       vout[pos] = v;
+      if (pl.xs) {
+        pl.xs[pos] = pl.x[v];
+        pl.ys[pos] = pl.y[v];
+        pl.zs[pos] = pl.z[v];
+        if (pl.qs) pl.qs[pos] = pl.q[v];
+      }
     }
     __syncthreads();
     for (int dd = threadIdx.x; dd <= int(mask); dd += RS_T) {
@@ -304,28 +316,36 @@ __device__ __forceinline__ void cic_terms(double x, double y, double z, double q
   }
 }
 
-constexpr uint32_t kLight = 48;
+// cells with <= kLight particles: one thread, particles in index order;
+// <= kMedium: one wave (lane-strided sums, fixed butterfly);
+// more: one workgroup (thread-strided sums, fixed LDS tree).
+constexpr uint32_t kLight = 16;
+constexpr uint32_t kMedium = 16384;
 
-__global__ void k_cell_reduce(const uint32_t* __restrict__ sv, const uint32_t* __restrict__ start,
-                              const uint32_t* __restrict__ end, const double* __restrict__ x,
-                              const double* __restrict__ y, const double* __restrict__ z,
-                              const double* __restrict__ q, CellParams p, double* __restrict__ S,
-                              uint32_t* __restrict__ heavy, uint32_t* __restrict__ nheavy) {
+__global__ void k_cell_reduce(const double* __restrict__ xs, const double* __restrict__ ys,
+                              const double* __restrict__ zs, const double* __restrict__ qs,
+                              const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                              CellParams p, double* __restrict__ S, uint32_t* __restrict__ med,
+                              uint32_t* __restrict__ nmed, uint32_t* __restrict__ heavy,
+                              uint32_t* __restrict__ nheavy) {
   const int64_t ncell = int64_t(p.nn[0]) * p.nn[1] * p.nn[2];
   for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < ncell;
        c += int64_t(gridDim.x) * blockDim.x) {
     const uint32_t st = start[c], en = end[c];
-    if (en - st > kLight) {
+    if (en - st > kMedium) {
       heavy[atomicAdd(nheavy, 1u)] = uint32_t(c);
+      continue;
+    }
+    if (en - st > kLight) {
+      med[atomicAdd(nmed, 1u)] = uint32_t(c);
       continue;
     }
     const int i0 = int(c % p.nn[0]), j0 = int((c / p.nn[0]) % p.nn[1]),
               k0 = int(c / (int64_t(p.nn[0]) * p.nn[1]));
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t e = st; e < en; ++e) {
-      const uint32_t pi = sv[e];
       double t[8];
-      cic_terms(x[pi], y[pi], z[pi], p.hasq ? q[pi] : p.q_each, p, i0, j0, k0, t);
+      cic_terms(xs[e], ys[e], zs[e], p.hasq ? qs[e] : p.q_each, p, i0, j0, k0, t);
 #pragma unroll
       for (int k = 0; k < 8; ++k) s[k] = s[k] + t[k];
     }
@@ -335,11 +355,44 @@ __global__ void k_cell_reduce(const uint32_t* __restrict__ sv, const uint32_t* _
 }
 
 __global__ void __launch_bounds__(kThreads)
-    k_cell_heavy(const uint32_t* __restrict__ sv, const uint32_t* __restrict__ start,
-                 const uint32_t* __restrict__ end, const double* __restrict__ x,
-                 const double* __restrict__ y, const double* __restrict__ z,
-                 const double* __restrict__ q, CellParams p, double* __restrict__ S,
-                 const uint32_t* __restrict__ heavy, const uint32_t* __restrict__ nheavy) {
+    k_cell_medium(const double* __restrict__ xs, const double* __restrict__ ys,
+                  const double* __restrict__ zs, const double* __restrict__ qs,
+                  const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                  CellParams p, double* __restrict__ S, const uint32_t* __restrict__ med,
+                  const uint32_t* __restrict__ nmed) {
+  const int64_t ncell = int64_t(p.nn[0]) * p.nn[1] * p.nn[2];
+  const uint32_t nm = *nmed;
+  const int lane = threadIdx.x & 63;
+  const uint32_t wv = (blockIdx.x * kThreads + threadIdx.x) >> 6;
+  const uint32_t nw = (gridDim.x * kThreads) >> 6;
+  for (uint32_t h = wv; h < nm; h += nw) {
+    const uint32_t c = med[h];
+    const uint32_t st = start[c], en = end[c];
+    const int i0 = int(c % p.nn[0]), j0 = int((c / p.nn[0]) % p.nn[1]),
+              k0 = int(c / (uint32_t(p.nn[0]) * p.nn[1]));
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t e = st + lane; e < en; e += 64) {
+      double t[8];
+      cic_terms(xs[e], ys[e], zs[e], p.hasq ? qs[e] : p.q_each, p, i0, j0, k0, t);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s[k] = s[k] + t[k];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s[k] = s[k] + __shfl_xor(s[k], off, 64);
+    if (lane == 0)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) S[k * ncell + c] = s[k];
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_cell_heavy(const double* __restrict__ xs, const double* __restrict__ ys,
+                 const double* __restrict__ zs, const double* __restrict__ qs,
+                 const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                 CellParams p, double* __restrict__ S, const uint32_t* __restrict__ heavy,
+                 const uint32_t* __restrict__ nheavy) {
   __shared__ double red[8][kThreads];
   const int64_t ncell = int64_t(p.nn[0]) * p.nn[1] * p.nn[2];
   const uint32_t nh = *nheavy;
@@ -350,9 +403,8 @@ __global__ void __launch_bounds__(kThreads)
               k0 = int(c / (uint32_t(p.nn[0]) * p.nn[1]));
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t e = st + threadIdx.x; e < en; e += kThreads) {
-      const uint32_t pi = sv[e];
       double t[8];
-      cic_terms(x[pi], y[pi], z[pi], p.hasq ? q[pi] : p.q_each, p, i0, j0, k0, t);
+      cic_terms(xs[e], ys[e], zs[e], p.hasq ? qs[e] : p.q_each, p, i0, j0, k0, t);
 #pragma unroll
       for (int k = 0; k < 8; ++k) s[k] = s[k] + t[k];
     }
@@ -399,6 +451,7 @@ void bbox(osc_ctx* c, int64_t n, const double* x, const double* y, const double*
   DevBuf part(c->pool, sizeof(double) * 6 * kRedBlocks + 64);
   double* pp = part.as<double>();
   int nb = int(std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + kThreads - 1) / kThreads)));
+  ProfScope ps(c, ST_DEP_BBOX);
   hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(kThreads), 0, c->stream, x, y, z, n, pp);
   OSC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kThreads), 0, c->stream, pp, nb,
@@ -448,28 +501,33 @@ void fill_u32(osc_ctx* c, uint32_t* p, int64_t n, uint32_t v) {
   OSC_HIP(hipGetLastError());
 }
 
-// returns the device buffers holding the sorted (key, value) pairs
+// Sorts (key, value) pairs; values start as the input index.  The last pass
+// also writes the payload (coordinates, charges) in sorted order.  When the
+// key range needs no pass the pairs are already "sorted" and the payload is
+// copied through a one-pass scatter with a zero-bit digit.
 static void radix_sort(osc_ctx* c, DevBuf& ka, DevBuf& va, DevBuf& kb, DevBuf& vb, int64_t n,
-                       int nbits, bool first_values_are_index, bool& result_in_b) {
-  const int npass = nbits <= 0 ? 0 : (nbits + 7) / 8;
+                       int nbits, const SortPayload& pl, bool& result_in_b) {
+  int npass = nbits <= 0 ? 0 : (nbits + 7) / 8;
+  if (npass == 0) npass = 1;
   const int nb = int((n + RS_TILE - 1) / RS_TILE);
   result_in_b = false;
-  if (npass == 0 || n == 0) return;
+  if (n == 0) return;
   DevBuf hist(c->pool, sizeof(uint32_t) * 256 * size_t(nb));
   DevBuf offs(c->pool, sizeof(uint32_t) * 256 * size_t(nb));
   uint32_t *kin = ka.as<uint32_t>(), *vin = va.as<uint32_t>();
   uint32_t *kout = kb.as<uint32_t>(), *vout = vb.as<uint32_t>();
   int shift = 0;
   for (int p = 0; p < npass; ++p) {
-    const int bits = (nbits - shift + (npass - p) - 1) / (npass - p);
+    const int bits = nbits > 0 ? (nbits - shift + (npass - p) - 1) / (npass - p) : 0;
     const uint32_t mask = (1u << bits) - 1u;
     hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_T), 0, c->stream, kin, n, shift, mask,
                        hist.as<uint32_t>(), nb);
     OSC_HIP(hipGetLastError());
     exclusive_scan(c, hist.as<const uint32_t>(), offs.as<uint32_t>(), int64_t(mask + 1) * nb);
+    SortPayload none{};
     hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_T), 0, c->stream, kin,
-                       (p == 0 && first_values_are_index) ? nullptr : vin, kout, vout, n, shift,
-                       bits, offs.as<const uint32_t>(), nb);
+                       p == 0 ? nullptr : vin, kout, vout, n, shift, bits,
+                       offs.as<const uint32_t>(), nb, p == npass - 1 ? pl : none);
     OSC_HIP(hipGetLastError());
     std::swap(kin, kout);
     std::swap(vin, vout);
@@ -478,16 +536,28 @@ static void radix_sort(osc_ctx* c, DevBuf& ka, DevBuf& va, DevBuf& kb, DevBuf& v
   }
 }
 
-int64_t deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho) {
+int64_t read_counter(osc_ctx* c, int idx) {
+  uint32_t v = 0;
+  OSC_HIP(hipMemcpyAsync(&v, static_cast<uint32_t*>(c->counters) + idx, sizeof(v),
+                         hipMemcpyDeviceToHost, c->stream));
+  OSC_HIP(hipStreamSynchronize(c->stream));
+  return int64_t(v);
+}
+
+void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles* keep) {
   const int nx = a.nn[0], ny = a.nn[1], nz = a.nn[2];
   const int64_t ncell = int64_t(nx) * ny * nz;
   OSC_REQUIRE(ncell < (int64_t(1) << 31) && a.n < (int64_t(1) << 32) - 1, OSC_ERR_ARG,
               "grid or particle count too large for 32-bit keys");
-  uint32_t* cnt = static_cast<uint32_t*>(c->counters);  // [0] dropped, [1] heavy
-  fill_u32(c, cnt, 2, 0u);
+  // counters: [0] dropped, [1] heavy cells, [3] medium cells
+  uint32_t* cnt = static_cast<uint32_t*>(c->counters);
+  fill_u32(c, cnt, 4, 0u);
   const int64_t n = a.n;
   DevBuf ka(c->pool, sizeof(uint32_t) * (n + 1)), va(c->pool, sizeof(uint32_t) * (n + 1));
   DevBuf kb(c->pool, sizeof(uint32_t) * (n + 1)), vb(c->pool, sizeof(uint32_t) * (n + 1));
+  DevBuf xs(c->pool, sizeof(double) * (n + 1)), ys(c->pool, sizeof(double) * (n + 1)),
+      zs(c->pool, sizeof(double) * (n + 1));
+  DevBuf qs(c->pool, a.q ? sizeof(double) * (n + 1) : 0);
   KeyParams kp{};
   CellParams cp{};
   for (int d = 0; d < 3; ++d) {
@@ -507,6 +577,7 @@ int64_t deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho) {
   cp.harness = a.harness;
   cp.q_each = a.q_each;
   if (n > 0) {
+    ProfScope ps(c, ST_DEP_KEYS);
     hipLaunchKernelGGL(k_cic_keys, dim3(nblk(n)), dim3(kThreads), 0, c->stream, a.x, a.y, a.z, n,
                        kp, ka.as<uint32_t>(), cnt);
     OSC_HIP(hipGetLastError());
@@ -514,9 +585,14 @@ int64_t deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho) {
   int nbits = 0;
   while ((uint64_t(1) << nbits) <= uint64_t(ncell)) ++nbits;
   bool in_b = false;
-  radix_sort(c, ka, va, kb, vb, n, nbits, true, in_b);
+  SortPayload pl{a.x, a.y, a.z, a.q, xs.as<double>(), ys.as<double>(), zs.as<double>(),
+                 a.q ? qs.as<double>() : nullptr};
+  {
+    ProfScope ps(c, ST_DEP_SORT);
+    radix_sort(c, ka, va, kb, vb, n, nbits, pl, in_b);
+  }
+  ProfScope ps_cells(c, ST_DEP_CELLS);
   const uint32_t* sk = in_b ? kb.as<uint32_t>() : ka.as<uint32_t>();
-  const uint32_t* sv = in_b ? vb.as<uint32_t>() : va.as<uint32_t>();
   DevBuf st(c->pool, sizeof(uint32_t) * ncell), en(c->pool, sizeof(uint32_t) * ncell);
   fill_u32(c, st.as<uint32_t>(), ncell, 0u);
   fill_u32(c, en.as<uint32_t>(), ncell, 0u);
@@ -525,22 +601,38 @@ int64_t deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho) {
                        st.as<uint32_t>(), en.as<uint32_t>());
     OSC_HIP(hipGetLastError());
   }
-  DevBuf S(c->pool, sizeof(double) * 8 * ncell), heavy(c->pool, sizeof(uint32_t) * ncell);
-  hipLaunchKernelGGL(k_cell_reduce, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream, sv,
-                     st.as<const uint32_t>(), en.as<const uint32_t>(), a.x, a.y, a.z, a.q, cp,
-                     S.as<double>(), heavy.as<uint32_t>(), cnt + 1);
+  DevBuf S(c->pool, sizeof(double) * 8 * ncell), heavy(c->pool, sizeof(uint32_t) * ncell),
+      med(c->pool, sizeof(uint32_t) * ncell);
+  const double* qsp = a.q ? qs.as<const double>() : nullptr;
+  hipLaunchKernelGGL(k_cell_reduce, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,
+                     xs.as<const double>(), ys.as<const double>(), zs.as<const double>(), qsp,
+                     st.as<const uint32_t>(), en.as<const uint32_t>(), cp, S.as<double>(),
+                     med.as<uint32_t>(), cnt + 3, heavy.as<uint32_t>(), cnt + 1);
   OSC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_cell_heavy, dim3(2048), dim3(kThreads), 0, c->stream, sv,
-                     st.as<const uint32_t>(), en.as<const uint32_t>(), a.x, a.y, a.z, a.q, cp,
-                     S.as<double>(), heavy.as<const uint32_t>(), cnt + 1);
+  hipLaunchKernelGGL(k_cell_medium, dim3(4096), dim3(kThreads), 0, c->stream,
+                     xs.as<const double>(), ys.as<const double>(), zs.as<const double>(), qsp,
+                     st.as<const uint32_t>(), en.as<const uint32_t>(), cp, S.as<double>(),
+                     med.as<const uint32_t>(), cnt + 3);
   OSC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_node, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,
-                     S.as<const double>(), nx, ny, nz, a.harness, a.scale_after, rho);
+  hipLaunchKernelGGL(k_cell_heavy, dim3(1024), dim3(kThreads), 0, c->stream,
+                     xs.as<const double>(), ys.as<const double>(), zs.as<const double>(), qsp,
+                     st.as<const uint32_t>(), en.as<const uint32_t>(), cp, S.as<double>(),
+                     heavy.as<const uint32_t>(), cnt + 1);
   OSC_HIP(hipGetLastError());
-  uint32_t nd = 0;
-  OSC_HIP(hipMemcpyAsync(&nd, cnt, sizeof(nd), hipMemcpyDeviceToHost, c->stream));
-  OSC_HIP(hipStreamSynchronize(c->stream));
-  return int64_t(nd);
+  ps_cells.stop();
+  {
+    ProfScope ps(c, ST_DEP_NODES);
+    hipLaunchKernelGGL(k_node, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,
+                       S.as<const double>(), nx, ny, nz, a.harness, a.scale_after, rho);
+    OSC_HIP(hipGetLastError());
+  }
+  if (keep) {
+    keep->n = n;
+    keep->sv = std::move(in_b ? vb : va);
+    keep->xs = std::move(xs);
+    keep->ys = std::move(ys);
+    keep->zs = std::move(zs);
+  }
 }
 
 }  // namespace osc

@@ -79,7 +79,110 @@ __global__ void k_interp(const double* __restrict__ ef, const double* __restrict
   __syncthreads();
   if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
 }
+// node records: rec[node*RS + 0..2] = E, [3..5] = B (RS = 8, 64-byte records)
+// so that the 8 nodes of a cell are 4 pairs of adjacent records
+__global__ void k_pack_nodes(const double* __restrict__ ef, const double* __restrict__ bf,
+                             int64_t nt, double* __restrict__ rec) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nt;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    double r[8];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      r[c] = ef ? ef[c * nt + i] : 0.0;
+      r[3 + c] = bf ? bf[c * nt + i] : 0.0;
+    }
+    r[6] = r[7] = 0.0;
+    double4* o = reinterpret_cast<double4*>(rec + 8 * i);
+    o[0] = make_double4(r[0], r[1], r[2], r[3]);
+    o[1] = make_double4(r[4], r[5], r[6], r[7]);
+  }
+}
+
+__device__ __forceinline__ double wsum8(const double* __restrict__ R, int64_t sy, int64_t sz,
+                                        int64_t b, int c, double ab, double de, double gh) {
+  // same 8 terms and order as interp8, reading records (stride 8 doubles)
+  const int64_t i1 = 8, j1 = 8 * sy, k1 = 8 * sz, o = 8 * b + c;
+  return R[o] * ab * de * gh + R[o + j1] * ab * (1. - de) * gh +
+         R[o + j1 + k1] * ab * (1. - de) * (1. - gh) + R[o + k1] * ab * de * (1. - gh) +
+         R[o + i1 + k1] * (1. - ab) * de * (1. - gh) +
+         R[o + i1 + j1 + k1] * (1. - ab) * (1. - de) * (1. - gh) +
+         R[o + i1 + j1] * (1. - ab) * (1. - de) * gh + R[o + i1] * (1. - ab) * de * gh;
+}
+
+// SORTED: slot s holds particle sv[s] with coordinates x[s], y[s], z[s]
+template <bool SORTED>
+__global__ void k_interp_rec(const double* __restrict__ R, int wantE, int wantB, int64_t n,
+                             const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ z, const uint32_t* __restrict__ sv, IP p,
+                             double* __restrict__ E, double* __restrict__ B,
+                             int8_t* __restrict__ clamped, uint32_t* __restrict__ nclamp) {
+  __shared__ uint32_t cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  uint32_t mine = 0;
+  const int64_t sy = p.nn[0], sz = int64_t(p.nn[0]) * p.nn[1];
+  for (int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; s < n;
+       s += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = SORTED ? int64_t(sv[s]) : s;
+    const double xi = x[s], yi = y[s], zi = z[s];
+    bool fl = false;
+    const int ip = clamp_idx(xi, p.mn[0], p.hinv[0], p.nhi[0], fl);
+    const int jp = clamp_idx(yi, p.mn[1], p.hinv[1], p.nhi[1], fl);
+    const int kp = clamp_idx(zi, p.mn[2], p.hinv[2], p.nhi[2], fl);
+    const double ab = ((p.mn[0] - xi) + ip * p.delta[0]) * p.hinv[0];
+    const double de = ((p.mn[1] - yi) + jp * p.delta[1]) * p.hinv[1];
+    const double gh = ((p.mn[2] - zi) + kp * p.delta[2]) * p.hinv[2];
+    int i0 = ip - p.nlo[0], j0 = jp - p.nlo[1], k0 = kp - p.nlo[2];
+    i0 = i0 < 0 ? 0 : (i0 > p.nn[0] - 2 ? p.nn[0] - 2 : i0);
+    j0 = j0 < 0 ? 0 : (j0 > p.nn[1] - 2 ? p.nn[1] - 2 : j0);
+    k0 = k0 < 0 ? 0 : (k0 > p.nn[2] - 2 ? p.nn[2] - 2 : k0);
+    const int64_t b = i0 + j0 * sy + k0 * sz;
+    if (wantE)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) E[3 * i + c] = wsum8(R, sy, sz, b, c, ab, de, gh);
+    if (wantB)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) B[3 * i + c] = wsum8(R, sy, sz, b, 3 + c, ab, de, gh);
+    if (clamped) clamped[i] = fl ? 1 : 0;
+    mine += fl ? 1u : 0u;
+  }
+  if (mine) atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
+}
 }  // namespace
+
+void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
+                        const SortedParticles& sp, double* E, double* B, int8_t* clamped) {
+  IP p{};
+  for (int d = 0; d < 3; ++d) {
+    p.mn[d] = a.mn[d];
+    p.delta[d] = a.delta[d];
+    p.hinv[d] = 1.0 / a.delta[d];
+    p.nlo[d] = a.nlo[d];
+    p.nhi[d] = a.nhi[d];
+    p.nn[d] = a.nn[d];
+    OSC_REQUIRE(a.nn[d] >= 2, OSC_ERR_ARG, "interpolation needs at least 2 nodes per axis");
+  }
+  const int64_t nt = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
+  uint32_t* cnt = static_cast<uint32_t*>(c->counters) + 2;
+  ProfScope ps(c, ST_GATHER);
+  fill_u32(c, cnt, 1, 0u);
+  DevBuf rec(c->pool, sizeof(double) * 8 * nt);
+  int64_t nb = std::min<int64_t>((nt + kThreads - 1) / kThreads, 65536 * 8);
+  hipLaunchKernelGGL(k_pack_nodes, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream, ef, bf, nt,
+                     rec.as<double>());
+  OSC_HIP(hipGetLastError());
+  const int64_t n = sp.n;
+  if (n > 0) {
+    nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
+    hipLaunchKernelGGL(k_interp_rec<true>, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
+                       rec.as<const double>(), int(E && ef), int(B && bf), n,
+                       sp.xs.as<const double>(), sp.ys.as<const double>(), sp.zs.as<const double>(),
+                       sp.sv.as<const uint32_t>(), p, E, B, clamped, cnt);
+    OSC_HIP(hipGetLastError());
+  }
+}
 
 int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
                     int64_t n, const double* x, const double* y, const double* z, double* E,
@@ -95,14 +198,28 @@ int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const dou
     OSC_REQUIRE(a.nn[d] >= 2, OSC_ERR_ARG, "interpolation needs at least 2 nodes per axis");
   }
   uint32_t* cnt = static_cast<uint32_t*>(c->counters) + 2;
+  ProfScope ps(c, ST_GATHER);
   fill_u32(c, cnt, 1, 0u);
-  if (n > 0) {
-    int64_t nb = (n + kThreads - 1) / kThreads;
-    if (nb > 65536 * 8) nb = 65536 * 8;
+  if (n <= 0) return 0;
+  const int64_t nt = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
+  if (n < nt / 8) {
+    // few points: read the planar fields directly instead of packing the mesh
+    int64_t nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
     hipLaunchKernelGGL(k_interp, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream, ef, bf, n, x,
                        y, z, p, E, B, clamped, cnt);
     OSC_HIP(hipGetLastError());
+    return 0;
   }
+  DevBuf rec(c->pool, sizeof(double) * 8 * nt);
+  int64_t nb = std::min<int64_t>((nt + kThreads - 1) / kThreads, 65536 * 8);
+  hipLaunchKernelGGL(k_pack_nodes, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream, ef, bf, nt,
+                     rec.as<double>());
+  OSC_HIP(hipGetLastError());
+  nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
+  hipLaunchKernelGGL(k_interp_rec<false>, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
+                     rec.as<const double>(), int(E && ef), int(B && bf), n, x, y, z, nullptr, p, E,
+                     B, clamped, cnt);
+  OSC_HIP(hipGetLastError());
   return 0;
 }
 

@@ -136,6 +136,7 @@ __global__ void k_convert(const cpx<double>* __restrict__ in, cpx<float>* __rest
 
 void build_green_real(osc_ctx* c, const GreenSpec& s, double* g) {
   const int64_t N = int64_t(s.L[0]) * s.L[1] * s.L[2];
+  ProfScope ps(c, ST_GREEN_EVAL);
   if (s.kind == GK_OFFSET) {
     // osc_get_cgrn_freespace (open_spacecharge_mod.f90:633-707)
     const double dx = s.delta[0], dy = s.delta[1], dz = s.delta[2] * s.gamma;
@@ -165,6 +166,7 @@ void build_green_real(osc_ctx* c, const GreenSpec& s, double* g) {
 
 void spectrum_forward_full(osc_ctx* c, const double* g, cpx<double>* spec, const SpecDims& d) {
   const int L0 = d.L[0], L1 = d.L[1], L2 = d.L[2];
+  ProfScope ps(c, ST_GREEN_FFT);
   R2CIO r{};
   r.in_sy = L0;
   r.in_sz = int64_t(L0) * L1;
@@ -195,6 +197,7 @@ void spectrum_forward_full(osc_ctx* c, const double* g, cpx<double>* spec, const
 }
 
 void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_t n) {
+  ProfScope ps(c, ST_GREEN_FFT);
   hipLaunchKernelGGL(k_convert, dim3(nblk(n)), dim3(kThreads), 0, c->stream, in, out, n);
   OSC_HIP(hipGetLastError());
 }

@@ -72,6 +72,32 @@ struct PipeState {
   DevBuf spec64[4], spec32[4];
 };
 
+// timed stages (osc_stage_name)
+enum Stage : int {
+  ST_DEP_BBOX = 0,
+  ST_DEP_KEYS,
+  ST_DEP_SORT,
+  ST_DEP_CELLS,
+  ST_DEP_NODES,
+  ST_GREEN_EVAL,
+  ST_GREEN_FFT,
+  ST_FWD_X,
+  ST_FWD_Y,
+  ST_FWD_Z,
+  ST_INV_ZMUL,
+  ST_INV_Y,
+  ST_INV_X,
+  ST_GATHER,
+  ST_FD_B,
+  ST_OTHER,
+  ST_COUNT
+};
+
+struct ProfRec {
+  int stage;
+  hipEvent_t a, b;
+};
+
 }  // namespace osc
 
 struct osc_ctx {
@@ -86,7 +112,47 @@ struct osc_ctx {
   osc::PipeState pipe;
   void* counters = nullptr;  // small device scratch for counters (persistent)
   void* host_pinned = nullptr;
+  bool prof = false;
+  std::vector<osc::ProfRec> prof_recs;
+  std::vector<hipEvent_t> prof_free;
+  double prof_ms[osc::ST_COUNT] = {0};
+  int64_t prof_n[osc::ST_COUNT] = {0};
 };
+
+namespace osc {
+// brackets the launches of one stage with events when profiling is on
+class ProfScope {
+ public:
+  ProfScope(osc_ctx* c, int stage) : c_(c->prof ? c : nullptr), stage_(stage) {
+    if (!c_) return;
+    a_ = take();
+    (void)hipEventRecord(a_, c_->stream);
+  }
+  ~ProfScope() { stop(); }
+  void stop() {
+    if (!c_) return;
+    hipEvent_t b = take();
+    (void)hipEventRecord(b, c_->stream);
+    c_->prof_recs.push_back({stage_, a_, b});
+    c_ = nullptr;
+  }
+
+ private:
+  hipEvent_t take() {
+    if (!c_->prof_free.empty()) {
+      hipEvent_t e = c_->prof_free.back();
+      c_->prof_free.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  osc_ctx* c_;
+  int stage_;
+  hipEvent_t a_ = nullptr;
+};
+}  // namespace osc
 
 namespace osc {
 
@@ -177,8 +243,16 @@ struct DepositArgs {
   int harness;          // 1: depose_rho_scalar rules (unit weights, scale at end)
   double scale_after;   // harness chrgpermacro
 };
-// rho (nx,ny,nz) on device; returns dropped count (synchronises)
-int64_t deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho);
+// particles in cell-sorted order, as left by a deposit (for a following gather)
+struct SortedParticles {
+  int64_t n = 0;
+  DevBuf sv;          // original particle index of each sorted slot (u32)
+  DevBuf xs, ys, zs;  // coordinates in sorted order
+};
+// rho (nx,ny,nz) on device; the dropped count is left in counter 0
+// (read_counter(c, 0)); keep (optional) receives the sorted particles
+void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles* keep = nullptr);
+int64_t read_counter(osc_ctx* c, int idx);  // synchronises
 void bbox(osc_ctx* c, int64_t n, const double* x, const double* y, const double* z, double lo[3],
           double hi[3]);
 double device_sum(osc_ctx* c, const double* v, int64_t n);
@@ -191,5 +265,8 @@ struct InterpArgs {
 int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
                     int64_t n, const double* x, const double* y, const double* z, double* E,
                     double* B, int8_t* clamped);
+// same, for the particles of the preceding deposit in their cell-sorted order
+void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
+                        const SortedParticles& sp, double* E, double* B, int8_t* clamped);
 
 }  // namespace osc

@@ -87,6 +87,7 @@ void fill_f64(osc_ctx* c, double* p, int64_t n, double v) {
 void fd_gradient(osc_ctx* c, const double* phi, const int n[3], const double delta[3], double gam,
                  double* ef) {
   const int64_t nt = int64_t(n[0]) * n[1] * n[2];
+  ProfScope ps(c, ST_FD_B);
   hipLaunchKernelGGL(k_fd, dim3(nblk(nt)), dim3(kThreads), 0, c->stream, phi, n[0], n[1], n[2],
                      delta[0], delta[1], delta[2], gam, ef);
   OSC_HIP(hipGetLastError());
@@ -94,6 +95,7 @@ void fd_gradient(osc_ctx* c, const double* phi, const int n[3], const double del
 
 void bfield_core(osc_ctx* c, const double* ef, int64_t n, double gam, double* bf) {
   const double beta0 = sqrt(1.0 - 1.0 / (gam * gam));
+  ProfScope ps(c, ST_FD_B);
   hipLaunchKernelGGL(k_bcore, dim3(nblk(n)), dim3(kThreads), 0, c->stream, ef, n, beta0,
                      kClightDp, bf);
   OSC_HIP(hipGetLastError());
@@ -131,7 +133,10 @@ void conv_solve(osc_ctx* c, const double* rho, const int n[3], const SpecDims& d
   r.nrows = int64_t(ny) * nz;
   r.ny_rows = ny;
   r.n_in = nx;
-  launch_xr2c<T>(L0, rho, sp, r, twiddles<T>(c, L0), s);
+  {
+    ProfScope ps(c, ST_FWD_X);
+    launch_xr2c<T>(L0, rho, sp, r, twiddles<T>(c, L0), s);
+  }
   AxisIO ay{};
   ay.s_b = ay.d_b = 1;
   ay.s_p = ay.d_p = KXP;
@@ -140,7 +145,10 @@ void conv_solve(osc_ctx* c, const double* rho, const int n[3], const SpecDims& d
   ay.n_in = ny;
   ay.w0 = 0;
   ay.n_out = L1;
-  launch_axis<T>(L1, -1, sp, sp, ay, nz, 1, twiddles<T>(c, L1), s);
+  {
+    ProfScope ps(c, ST_FWD_Y);
+    launch_axis<T>(L1, -1, sp, sp, ay, nz, 1, twiddles<T>(c, L1), s);
+  }
   AxisIO az{};
   az.s_b = az.d_b = 1;
   az.s_p = az.d_p = KXP * L1;
@@ -149,7 +157,10 @@ void conv_solve(osc_ctx* c, const double* rho, const int n[3], const SpecDims& d
   az.n_in = nz;
   az.w0 = 0;
   az.n_out = L2;
-  launch_axis<T>(L2, -1, sp, sp, az, L1, 1, twiddles<T>(c, L2), s);
+  {
+    ProfScope ps(c, ST_FWD_Z);
+    launch_axis<T>(L2, -1, sp, sp, az, L1, 1, twiddles<T>(c, L2), s);
+  }
   // inverse: combine + z, y (window), x (window + epilogue)
   const int nout = int(outs.size());
   const int64_t so = KXP * L1 * nz;
@@ -174,7 +185,10 @@ void conv_solve(osc_ctx* c, const double* rho, const int n[3], const SpecDims& d
       zi.out[o].t[t].mir = outs[o].t[t].mir;
     }
   }
-  launch_zinv<T>(L2, sp, tp, zi, L1, nout, twiddles<T>(c, L2), s);
+  {
+    ProfScope ps(c, ST_INV_ZMUL);
+    launch_zinv<T>(L2, sp, tp, zi, L1, nout, twiddles<T>(c, L2), s);
+  }
   AxisIO ai{};
   ai.s_b = ai.d_b = 1;
   ai.s_p = ai.d_p = KXP;
@@ -184,7 +198,10 @@ void conv_solve(osc_ctx* c, const double* rho, const int n[3], const SpecDims& d
   ai.n_in = L1;
   ai.w0 = win[1];
   ai.n_out = ny;
-  launch_axis<T>(L1, 1, tp, tp, ai, nz, nout, twiddles<T>(c, L1), s);
+  {
+    ProfScope ps(c, ST_INV_Y);
+    launch_axis<T>(L1, 1, tp, tp, ai, nz, nout, twiddles<T>(c, L1), s);
+  }
   C2RIO<T> cr{};
   cr.in_sy = KXP;
   cr.in_sz = KXP * L1;
@@ -196,7 +213,10 @@ void conv_solve(osc_ctx* c, const double* rho, const int n[3], const SpecDims& d
   cr.w0 = win[0];
   cr.n_out = nx;
   for (int o = 0; o < nout; ++o) cr.ep[o] = outs[o].ep;
-  launch_xc2r<T>(L0, tp, cr, nout, twiddles<T>(c, L0), s);
+  {
+    ProfScope ps(c, ST_INV_X);
+    launch_xc2r<T>(L0, tp, cr, nout, twiddles<T>(c, L0), s);
+  }
 }
 template void conv_solve<double>(osc_ctx*, const double*, const int[3], const SpecDims&,
                                  const int[3], const std::vector<OutSpec>&);

@@ -347,3 +347,41 @@ def test_fp32_convolution_error(oracle, nhi):
     err = relcomp(m.efield, e)
     assert err < 1e-5, err
     c32.close()
+
+
+# ---------------------------------------------------------------------------
+# fused tracking step == the three separate calls
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("at_cathode", [False, True])
+@pytest.mark.parametrize("use_q", [False, True])
+def test_space_charge_step_equals_separate_calls(ctx, at_cathode, use_q):
+    x, y, z = gaussian_bunch(40000, (1e-3, 1e-3, 5e-4), seed=31, center=(0, 0, 3e-3))
+    q = np.random.default_rng(8).uniform(0.5, 1.5, x.size) * 2.5e-14 if use_q else None
+    tc = None if use_q else 1e-9
+    m1 = osc.Mesh3D(nhi=(16, 16, 24), gamma=4.0)
+    osc.deposit_particles(x, y, z, m1, qa=q, total_charge=tc, ctx=ctx)
+    osc.space_charge_3d(m1, at_cathode=at_cathode, calc_bfield=True, ctx=ctx)
+    E1, B1 = osc.interpolate_field(x, y, z, m1, E=True, B=True, ctx=ctx)
+    m2 = osc.Mesh3D(nhi=(16, 16, 24), gamma=4.0)
+    E2, B2 = osc.space_charge_step(x, y, z, m2, qa=q, total_charge=tc, at_cathode=at_cathode,
+                                   calc_bfield=True, ctx=ctx)
+    assert np.array_equal(m1.rho, m2.rho)
+    assert np.array_equal(m1.efield, m2.efield) and np.array_equal(m1.bfield, m2.bfield)
+    assert np.array_equal(E1, E2) and np.array_equal(B1, B2)
+
+
+def test_interpolate_few_points_planar_path(ctx, oracle):
+    # fewer points than nodes/8 take the planar-field kernel
+    rng = np.random.default_rng(4)
+    m = osc.Mesh3D(nhi=(24, 20, 28), min=np.array([-1e-3, -2e-3, 0.5e-3]),
+                   delta=np.array([1e-4, 2e-4, 5e-5]))
+    m.allocate()
+    m.efield[...] = rng.standard_normal(m.efield.shape)
+    m.bfield[...] = rng.standard_normal(m.bfield.shape)
+    x, y, z = (rng.uniform(-1.1e-3, 1.4e-3, 37), rng.uniform(-2.1e-3, 2.0e-3, 37),
+               rng.uniform(0.4e-3, 1.9e-3, 37))
+    E, B = osc.interpolate_field(x, y, z, m, E=True, B=True, ctx=ctx)
+    Er, Br, _ = oracle.interpolate_field(m.nlo, m.nhi, m.min, m.delta, m.efield, m.bfield, x, y, z)
+    assert np.array_equal(E, Er.T) and np.array_equal(B, Br.T)
+    e0 = osc.interpolate_field(x[0], y[0], z[0], m, ctx=ctx)  # scalar call returns E(3)
+    assert e0.shape == (3,) and np.array_equal(e0, Er[:, 0])
