@@ -230,6 +230,22 @@ class Oracle:
                                                _d(phi), _d(e), _d(b))
         return rc, phi, e, b
 
+    def write_rectpipe_grn(self, path, a, b, delta, umin, umax, nlo, nhi, npad, gamma, greens):
+        per = self.legacy_period(nlo, nhi, npad)
+        gs = [np.require(g, dtype=np.complex128, requirements=["F"]) for g in greens]
+        arr = (_dp * 4)(*[g.ctypes.data_as(_dp) for g in gs])
+        return self.lib.oc_write_rectpipe_grn(os.fsencode(path), C.c_double(a), C.c_double(b),
+                                              _d3(delta), _d3(umin), _d3(umax), _i3(nlo), _i3(nhi),
+                                              _i3(per), C.c_double(gamma), arr)
+
+    def read_rectpipe_grn(self, path, nlo, nhi, npad):
+        per = self.legacy_period(nlo, nhi, npad)
+        gs = [np.zeros(per, dtype=np.complex128, order="F") for _ in range(4)]
+        arr = (_dp * 4)(*[g.ctypes.data_as(_dp) for g in gs])
+        hr, hi = (C.c_double * 12)(), (C.c_int * 9)()
+        rc = self.lib.oc_read_rectpipe_grn(os.fsencode(path), hr, hi, arr, C.c_int64(int(np.prod(per))))
+        return rc, np.array(hr[:]), np.array(hi[:]), gs
+
     # -- gather ------------------------------------------------------------------
     def interpolate_field(self, nlo, nhi, mn, delta, efield, bfield, x, y, z):
         x, y, z = _f64(x), _f64(y), _f64(z)
