@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--grid", type=int, default=NX)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-timeout", type=float, default=240.0)
+    p.add_argument("--cpu-child", nargs=2, metavar=("IN", "OUT"), help=argparse.SUPPRESS)
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run a 1 GiB device copy (profiler counter calibration)")
     return p.parse_args()
 
 
@@ -139,6 +142,7 @@ def gpu_leg(args, x, y, z):
         torch.cuda.synchronize()
         tc1 = time.perf_counter()
         fields[prec_name] = (mesh.efield.cpu().numpy().copy(order="F"), mesh.rho.cpu().numpy())
+        log(f"{prec_name}: {1e3 * (t1 - t0) / args.steps:.3f} ms/step")
         res[prec_name] = dict(
             solves_per_s=args.steps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / args.steps,
             p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
@@ -158,10 +162,42 @@ def gpu_leg(args, x, y, z):
     return res, fields
 
 
-def cpu_leg(args, x, y, z, mesh_gpu):
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_leg(args, x, y, z):
+    """Runs cpu_child() in a child process started before this process touches
+    the GPU, with the stack limit raised (upstream's FFT transposes keep large
+    automatic arrays on the stack).  Returns (cpu_baseline dict, E, deposit)."""
+    import resource
+    import subprocess
+    import tempfile
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    if hard == resource.RLIM_INFINITY or hard > soft:
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+    with tempfile.TemporaryDirectory() as td:
+        inp, out = os.path.join(td, "bunch.npz"), os.path.join(td, "cpu.npz")
+        np.savez(inp, x=x, y=y, z=z)
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", inp, out, "--grid",
+               str(args.grid)]
+        r = subprocess.run(cmd, timeout=args.cpu_timeout, capture_output=True, text=True)
+        if r.returncode != 0:
+            log(f"cpu leg failed (exit {r.returncode}): {r.stderr[-2000:]}")
+            return None, None, None
+        info = json.loads(r.stdout.strip().splitlines()[-1])
+        with np.load(out, allow_pickle=False) as f:
+            e = f["e"]
+            d = {k: f[k] for k in ("rho", "min", "max", "delta")}
+    return info, e, d
+
+
+def cpu_child(args):
     """Upstream Fortran library (oracle/_ref) on the host: one full step of the
-    same workload; falls back to the C restatement if it is not built."""
-    import ctypes as C
+    same workload; the C restatement if it is not built."""
+    inp, out = args.cpu_child
+    with np.load(inp, allow_pickle=False) as f:
+        x, y, z = f["x"], f["y"], f["z"]
     from oracle.oracle import REF_SO, Reference, Oracle
     ncore = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     nlo, nhi = (1, 1, 1), (args.grid,) * 3
@@ -173,10 +209,10 @@ def cpu_leg(args, x, y, z, mesh_gpu):
         e, b = R.space_charge_3d(d["rho"], nlo, nhi, d["min"], d["max"], d["delta"], GAMMA,
                                  calc_bfield=True)
         t2 = time.perf_counter()
-        mesh = dict(min=d["min"], max=d["max"], delta=d["delta"])
-        Ep, Bp = R.interpolate_field(nlo, nhi, d["min"], d["max"], d["delta"], e, b, x, y, z)
+        R.interpolate_field(nlo, nhi, d["min"], d["max"], d["delta"], e, b, x, y, z)
         t3 = time.perf_counter()
         kind = "reference"
+        what = "upstream Fortran library (amdflang -O2 -fopenmp; OpenMP only in its Green loops)"
     else:
         O = Oracle()
         t0 = time.perf_counter()
@@ -188,22 +224,53 @@ def cpu_leg(args, x, y, z, mesh_gpu):
         t3 = time.perf_counter()
         kind = "port"
         ncore = 1
+        what = "C restatement of the upstream algorithm, one thread"
     total = t3 - t0
-    return dict(value=1.0 / total, unit="solves/s", cores=ncore, kind=kind,
-                sample=f"one full step of the same workload ({args.grid}^3 mesh, {x.size} particles: "
-                       f"deposit {t1 - t0:.2f} s, solve+B {t2 - t1:.2f} s, gather E,B to all "
-                       f"particles {t3 - t2:.2f} s), FP64, OMP threads={ncore}"), e, d
+    info = dict(value=1.0 / total, unit="solves/s", cores=ncore, kind=kind,
+                sample=f"one full step of the same workload ({args.grid}^3 mesh, {x.size} particles): "
+                       f"deposit {t1 - t0:.2f} s, solve E+B {t2 - t1:.2f} s, gather E,B to all "
+                       f"particles {t3 - t2:.2f} s; {what}, OMP threads={ncore}")
+    np.savez(out, e=e, rho=d["rho"], min=d["min"], max=d["max"], delta=d["delta"])
+    print(json.dumps(info))
+
+
+def calibrate_copy():
+    """A plain device copy of known size, for scaling the FETCH_SIZE and
+    WRITE_SIZE counters of the same profiler run to bytes."""
+    import torch
+    nbytes = 1 << 30
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    for _ in range(4):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "copies": 4}))
 
 
 def main():
     args = parse()
+    if args.cpu_child:
+        cpu_child(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent solves "
               "shard across GPUs with no exchange, so the whole-job rate is N x the 1-GPU value.",
               file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     x, y, z = bunch(args.particles)
+    log(f"bunch ready: {x.size} particles")
+    cpu = e_ref = d_ref = None
+    if not args.no_cpu:
+        # before any GPU work: the child is started from a process with no GPU state
+        t = time.perf_counter()
+        cpu, e_ref, d_ref = cpu_leg(args, x, y, z)
+        log(f"cpu leg done in {time.perf_counter() - t:.1f} s")
     res, fields = gpu_leg(args, x, y, z)
+    log("gpu legs done")
     r64 = res["fp64"]
     # roofline of the dominant memory-bound stage
     n3 = (args.grid,) * 3
@@ -241,8 +308,7 @@ def main():
                      "traffic": None, "algorithmic_bytes_per_launch": alg,
                      "avg_launch_ms": round(per_launch_s * 1e3, 5)},
     }
-    if not args.no_cpu:
-        cpu, e_ref, d_ref = cpu_leg(args, x, y, z, r64["mesh"])
+    if cpu is not None:
         out["cpu_baseline"] = cpu
         # full-size FP64 parity against the upstream library on the same bunch
         e64, rho64 = fields["fp64"]

@@ -72,21 +72,33 @@ def build_hip(jobs: int | None = None, quiet: bool = True) -> str:
     return HIP_LIB
 
 
-def build_fortran(quiet: bool = True) -> str | None:
+F_SOURCES = ["opensc_hip.f90", "open_spacecharge_mod.f90"]
+F_DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "drive_shim.f90")
+F_DRIVER = os.path.join(BUILD, "drive_shim")
+
+
+def build_fortran(quiet: bool = True, driver: bool = True) -> str | None:
+    """The Fortran drop-in (libopensc_f.so, .mod files in fortran/include) and,
+    with driver, the test program tests/fortran/drive_shim.f90 linked to it."""
     if not os.path.exists(FLANG):
-        return None
-    src = os.path.join(FSRC, "open_spacecharge_mod.f90")
-    if not os.path.exists(src):
         return None
     os.makedirs(F_MODDIR, exist_ok=True)
     os.makedirs(BUILD, exist_ok=True)
-    obj = os.path.join(BUILD, "open_spacecharge_mod_f.o")
-    _run([FLANG, "-O2", "-fPIC", "-cpp", "-ffree-form", "-module-dir", F_MODDIR, "-c", src,
-          "-o", obj], quiet)
+    objs = []
+    for src in F_SOURCES:
+        obj = os.path.join(BUILD, src.replace(".f90", "_f.o"))
+        _run([FLANG, "-O2", "-fPIC", "-cpp", "-ffree-form", "-module-dir", F_MODDIR, "-c",
+              os.path.join(FSRC, src), "-o", obj], quiet)
+        objs.append(obj)
     tmp = F_LIB + ".tmp"
-    _run([FLANG, "-shared", "-fPIC", "-o", tmp, obj, f"-L{PKG}", "-lopensc_hip",
+    _run([FLANG, "-shared", "-fPIC", "-o", tmp, *objs, f"-L{PKG}", "-lopensc_hip",
           f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN"], quiet)
     os.replace(tmp, F_LIB)
+    if driver and os.path.exists(F_DRIVER_SRC):
+        _run([FLANG, "-O2", "-cpp", "-ffree-form", "-I", F_MODDIR, "-module-dir", BUILD,
+              F_DRIVER_SRC, "-o", F_DRIVER + ".tmp", f"-L{PKG}", "-lopensc_f", "-lopensc_hip",
+              f"-Wl,-rpath,{PKG}"], quiet)
+        os.replace(F_DRIVER + ".tmp", F_DRIVER)
     return F_LIB
 
 

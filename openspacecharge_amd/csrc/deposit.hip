@@ -4,8 +4,8 @@
 //
 //   1. cell key per particle: the upstream FP64 index formula, no contraction
 //   2. stable LSD radix sort of (cell, particle) pairs (wave-ballot ranking)
-//   3. per-cell partial sums of the 8 corner weights in particle order
-//      (one thread per light cell, one workgroup tree per heavy cell)
+//   3. wavefront-segmented reduction of the 8 corner terms over runs of
+//      equal cell keys in the sorted array (fixed shuffle and tree orders)
 //   4. each node sums the <= 8 cell/corner partials that touch it, in a fixed
 //      order.
 // No floating-point atomics: the result is bit-identical run to run.
@@ -264,20 +264,9 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
 }
 
 // ---------------------------------------------------------------------------
-// step 3: cell ranges and per-cell corner sums
+// step 3: wavefront-segmented reduction of the corner terms over runs of
+// equal cell keys in the sorted particle array
 // ---------------------------------------------------------------------------
-__global__ void k_cell_ranges(const uint32_t* __restrict__ ks, int64_t n,
-                              const uint32_t* __restrict__ ndrop, uint32_t* __restrict__ start,
-                              uint32_t* __restrict__ end) {
-  const int64_t nv = n - int64_t(*ndrop);
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nv;
-       i += int64_t(gridDim.x) * blockDim.x) {
-    const uint32_t k = ks[i];
-    if (i == 0 || ks[i - 1] != k) start[k] = uint32_t(i);
-    if (i == nv - 1 || ks[i + 1] != k) end[k] = uint32_t(i + 1);
-  }
-}
-
 struct CellParams {
   double mn[3], d[3], dinv[3];
   int wofs[3];
@@ -316,108 +305,147 @@ __device__ __forceinline__ void cic_terms(double x, double y, double z, double q
   }
 }
 
-// cells with <= kLight particles: one thread, particles in index order;
-// <= kMedium: one wave (lane-strided sums, fixed butterfly);
-// more: one workgroup (thread-strided sums, fixed LDS tree).
-constexpr uint32_t kLight = 16;
-constexpr uint32_t kMedium = 16384;
-
-__global__ void k_cell_reduce(const double* __restrict__ xs, const double* __restrict__ ys,
-                              const double* __restrict__ zs, const double* __restrict__ qs,
-                              const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                              CellParams p, double* __restrict__ S, uint32_t* __restrict__ med,
-                              uint32_t* __restrict__ nmed, uint32_t* __restrict__ heavy,
-                              uint32_t* __restrict__ nheavy) {
-  const int64_t ncell = int64_t(p.nn[0]) * p.nn[1] * p.nn[2];
-  for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < ncell;
-       c += int64_t(gridDim.x) * blockDim.x) {
-    const uint32_t st = start[c], en = end[c];
-    if (en - st > kMedium) {
-      heavy[atomicAdd(nheavy, 1u)] = uint32_t(c);
-      continue;
-    }
-    if (en - st > kLight) {
-      med[atomicAdd(nmed, 1u)] = uint32_t(c);
-      continue;
-    }
-    const int i0 = int(c % p.nn[0]), j0 = int((c / p.nn[0]) % p.nn[1]),
-              k0 = int(c / (int64_t(p.nn[0]) * p.nn[1]));
-    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t e = st; e < en; ++e) {
-      double t[8];
-      cic_terms(xs[e], ys[e], zs[e], p.hasq ? qs[e] : p.q_each, p, i0, j0, k0, t);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) s[k] = s[k] + t[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) S[k * ncell + c] = s[k];
-  }
-}
+// Each lane takes one sorted particle; a segmented inclusive scan over equal
+// keys (Hillis-Steele, fixed shuffle pattern) leaves each run's sum within
+// the wave at its last lane.  A run that starts and ends inside the wave is
+// written to S directly.  A run crossing a wave boundary leaves partials:
+// head[w] = its part in wave w when it began in an earlier wave and ends in w,
+// tail[w] = its part from its start in w (or lane 0) to lane 63 when it
+// continues into w+1.  k_seg_fix then adds, for each such run,
+// tail[first] + tail[middle waves...] + head[last] in wave order (a fixed
+// workgroup tree for runs longer than kSegSerial waves).  The summation order
+// depends only on the sorted order: deterministic, no atomics on data.
+constexpr int kSegSerial = 64;
 
 __global__ void __launch_bounds__(kThreads)
-    k_cell_medium(const double* __restrict__ xs, const double* __restrict__ ys,
-                  const double* __restrict__ zs, const double* __restrict__ qs,
-                  const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                  CellParams p, double* __restrict__ S, const uint32_t* __restrict__ med,
-                  const uint32_t* __restrict__ nmed) {
+    k_seg_reduce(const uint32_t* __restrict__ ks, const double* __restrict__ xs,
+                 const double* __restrict__ ys, const double* __restrict__ zs,
+                 const double* __restrict__ qs, int64_t n, const uint32_t* __restrict__ ndrop,
+                 CellParams p, double* __restrict__ S, double* __restrict__ head,
+                 double* __restrict__ tail) {
+  const int64_t nv = n - int64_t(*ndrop);
   const int64_t ncell = int64_t(p.nn[0]) * p.nn[1] * p.nn[2];
-  const uint32_t nm = *nmed;
   const int lane = threadIdx.x & 63;
-  const uint32_t wv = (blockIdx.x * kThreads + threadIdx.x) >> 6;
-  const uint32_t nw = (gridDim.x * kThreads) >> 6;
-  for (uint32_t h = wv; h < nm; h += nw) {
-    const uint32_t c = med[h];
-    const uint32_t st = start[c], en = end[c];
-    const int i0 = int(c % p.nn[0]), j0 = int((c / p.nn[0]) % p.nn[1]),
-              k0 = int(c / (uint32_t(p.nn[0]) * p.nn[1]));
-    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t e = st + lane; e < en; e += 64) {
-      double t[8];
-      cic_terms(xs[e], ys[e], zs[e], p.hasq ? qs[e] : p.q_each, p, i0, j0, k0, t);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) s[k] = s[k] + t[k];
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  // the loop bound is uniform per block (blockDim is a multiple of 64)
+  for (int64_t base = int64_t(blockIdx.x) * blockDim.x; base < nv; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    const int64_t w = i >> 6;
+    const bool valid = i < nv;
+    const uint32_t k = valid ? ks[i] : 0xFFFFFFFFu;
+    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (valid) {
+      const int i0 = int(k % uint32_t(p.nn[0]));
+      const int j0 = int((k / uint32_t(p.nn[0])) % uint32_t(p.nn[1]));
+      const int k0 = int(k / (uint32_t(p.nn[0]) * uint32_t(p.nn[1])));
+      cic_terms(xs[i], ys[i], zs[i], p.hasq ? qs[i] : p.q_each, p, i0, j0, k0, t);
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t kd = __shfl_up(k, d, 64);
+      double o[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) s[k] = s[k] + __shfl_xor(s[k], off, 64);
-    if (lane == 0)
+      for (int c = 0; c < 8; ++c) o[c] = __shfl_up(t[c], d, 64);
+      if (lane >= d && kd == k)
 #pragma unroll
-      for (int k = 0; k < 8; ++k) S[k * ncell + c] = s[k];
+        for (int c = 0; c < 8; ++c) t[c] = o[c] + t[c];
+    }
+    const uint32_t kn = __shfl_down(k, 1, 64);
+    const uint32_t kfirst = __shfl(k, 0, 64);
+    if (!valid || (lane < 63 && kn == k)) continue;  // not the last lane of its run
+    const int64_t w0 = w << 6;
+    const bool started = kfirst != k || w0 == 0 || ks[w0 - 1] != k;
+    const bool ends = lane < 63 || i + 1 >= nv || ks[i + 1] != k;
+    if (started && ends) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) S[c * ncell + k] = t[c];
+    } else if (ends) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) head[w * 8 + c] = t[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) tail[w * 8 + c] = t[c];
+    }
   }
 }
 
+// first index in [lo, hi) with ks[index] > k (ks sorted)
+__device__ __forceinline__ int64_t upper_bound_u32(const uint32_t* __restrict__ ks, int64_t lo,
+                                                   int64_t hi, uint32_t k) {
+  while (lo < hi) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (ks[mid] <= k) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// One thread per wave whose last run continues into the next wave and began
+// in this wave: completes that run.  Runs spanning more than kSegSerial waves
+// are listed for k_seg_heavy.
+__global__ void k_seg_fix(const uint32_t* __restrict__ ks, int64_t n,
+                          const uint32_t* __restrict__ ndrop, int64_t ncell,
+                          const double* __restrict__ head, const double* __restrict__ tail,
+                          double* __restrict__ S, uint32_t* __restrict__ heavy,
+                          uint32_t* __restrict__ nheavy) {
+  const int64_t nv = n - int64_t(*ndrop);
+  const int64_t nw = (nv + 63) >> 6;
+  for (int64_t w = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; w < nw;
+       w += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t last = (w << 6) + 63;
+    if (last + 1 >= nv) continue;
+    const uint32_t k = ks[last];
+    if (ks[last + 1] != k) continue;                          // no continuation
+    if (ks[w << 6] == k && w > 0 && ks[(w << 6) - 1] == k) continue;  // began earlier
+    const int64_t e = upper_bound_u32(ks, last + 1, nv, k);   // one past the run
+    const int64_t w1 = (e - 1) >> 6;
+    if (w1 - w > kSegSerial) {
+      heavy[atomicAdd(nheavy, 1u)] = uint32_t(w);
+      continue;
+    }
+    double acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = tail[w * 8 + c];
+    for (int64_t ww = w + 1; ww < w1; ++ww)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[c] = acc[c] + tail[ww * 8 + c];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) S[c * ncell + k] = acc[c] + head[w1 * 8 + c];
+  }
+}
+
+// One workgroup per long run: thread t sums the middle-wave partials
+// w+1+t, w+1+t+kThreads, ... in order, then a fixed LDS tree.
 __global__ void __launch_bounds__(kThreads)
-    k_cell_heavy(const double* __restrict__ xs, const double* __restrict__ ys,
-                 const double* __restrict__ zs, const double* __restrict__ qs,
-                 const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                 CellParams p, double* __restrict__ S, const uint32_t* __restrict__ heavy,
-                 const uint32_t* __restrict__ nheavy) {
+    k_seg_heavy(const uint32_t* __restrict__ ks, int64_t n, const uint32_t* __restrict__ ndrop,
+                int64_t ncell, const double* __restrict__ head, const double* __restrict__ tail,
+                double* __restrict__ S, const uint32_t* __restrict__ heavy,
+                const uint32_t* __restrict__ nheavy) {
   __shared__ double red[8][kThreads];
-  const int64_t ncell = int64_t(p.nn[0]) * p.nn[1] * p.nn[2];
+  const int64_t nv = n - int64_t(*ndrop);
   const uint32_t nh = *nheavy;
   for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
-    const uint32_t c = heavy[h];
-    const uint32_t st = start[c], en = end[c];
-    const int i0 = int(c % p.nn[0]), j0 = int((c / p.nn[0]) % p.nn[1]),
-              k0 = int(c / (uint32_t(p.nn[0]) * p.nn[1]));
+    const int64_t w = heavy[h];
+    const int64_t last = (w << 6) + 63;
+    const uint32_t k = ks[last];
+    const int64_t w1 = (upper_bound_u32(ks, last + 1, nv, k) - 1) >> 6;
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t e = st + threadIdx.x; e < en; e += kThreads) {
-      double t[8];
-      cic_terms(xs[e], ys[e], zs[e], p.hasq ? qs[e] : p.q_each, p, i0, j0, k0, t);
+    for (int64_t ww = w + 1 + threadIdx.x; ww < w1; ww += kThreads)
 #pragma unroll
-      for (int k = 0; k < 8; ++k) s[k] = s[k] + t[k];
-    }
+      for (int c = 0; c < 8; ++c) s[c] = s[c] + tail[ww * 8 + c];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = s[k];
+    for (int c = 0; c < 8; ++c) red[c][threadIdx.x] = s[c];
     __syncthreads();
-    for (int w = kThreads / 2; w > 0; w >>= 1) {
-      if (int(threadIdx.x) < w)
+    for (int m = kThreads / 2; m > 0; m >>= 1) {
+      if (int(threadIdx.x) < m)
 #pragma unroll
-        for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = red[k][threadIdx.x] + red[k][threadIdx.x + w];
+        for (int c = 0; c < 8; ++c) red[c][threadIdx.x] = red[c][threadIdx.x] + red[c][threadIdx.x + m];
       __syncthreads();
     }
-    if (threadIdx.x < 8) S[threadIdx.x * ncell + c] = red[threadIdx.x][0];
+    if (threadIdx.x < 8) {
+      const int c = threadIdx.x;
+      S[c * ncell + k] = (tail[w * 8 + c] + red[c][0]) + head[w1 * 8 + c];
+    }
     __syncthreads();
   }
 }
@@ -549,7 +577,7 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   const int64_t ncell = int64_t(nx) * ny * nz;
   OSC_REQUIRE(ncell < (int64_t(1) << 31) && a.n < (int64_t(1) << 32) - 1, OSC_ERR_ARG,
               "grid or particle count too large for 32-bit keys");
-  // counters: [0] dropped, [1] heavy cells, [3] medium cells
+  // counters: [0] dropped particles, [1] long runs (heavy cells); [2] is the gather's clamp count
   uint32_t* cnt = static_cast<uint32_t*>(c->counters);
   fill_u32(c, cnt, 4, 0u);
   const int64_t n = a.n;
@@ -593,32 +621,27 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   }
   ProfScope ps_cells(c, ST_DEP_CELLS);
   const uint32_t* sk = in_b ? kb.as<uint32_t>() : ka.as<uint32_t>();
-  DevBuf st(c->pool, sizeof(uint32_t) * ncell), en(c->pool, sizeof(uint32_t) * ncell);
-  fill_u32(c, st.as<uint32_t>(), ncell, 0u);
-  fill_u32(c, en.as<uint32_t>(), ncell, 0u);
+  // corner sums of every cell (zero where no particle), wave partials of runs
+  // crossing wave boundaries, list of long runs
+  const int64_t nw = (n + 63) / 64 + 1;
+  DevBuf S(c->pool, sizeof(double) * 8 * ncell), head(c->pool, sizeof(double) * 8 * nw),
+      tail(c->pool, sizeof(double) * 8 * nw), heavy(c->pool, sizeof(uint32_t) * nw);
+  fill_f64(c, S.as<double>(), 8 * ncell, 0.0);
+  const double* qsp = a.q ? qs.as<const double>() : nullptr;
   if (n > 0) {
-    hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk, n, cnt,
-                       st.as<uint32_t>(), en.as<uint32_t>());
+    hipLaunchKernelGGL(k_seg_reduce, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk,
+                       xs.as<const double>(), ys.as<const double>(), zs.as<const double>(), qsp, n,
+                       cnt, cp, S.as<double>(), head.as<double>(), tail.as<double>());
+    OSC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_seg_fix, dim3(nblk(nw)), dim3(kThreads), 0, c->stream, sk, n, cnt, ncell,
+                       head.as<const double>(), tail.as<const double>(), S.as<double>(),
+                       heavy.as<uint32_t>(), cnt + 1);
+    OSC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_seg_heavy, dim3(256), dim3(kThreads), 0, c->stream, sk, n, cnt, ncell,
+                       head.as<const double>(), tail.as<const double>(), S.as<double>(),
+                       heavy.as<const uint32_t>(), cnt + 1);
     OSC_HIP(hipGetLastError());
   }
-  DevBuf S(c->pool, sizeof(double) * 8 * ncell), heavy(c->pool, sizeof(uint32_t) * ncell),
-      med(c->pool, sizeof(uint32_t) * ncell);
-  const double* qsp = a.q ? qs.as<const double>() : nullptr;
-  hipLaunchKernelGGL(k_cell_reduce, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,
-                     xs.as<const double>(), ys.as<const double>(), zs.as<const double>(), qsp,
-                     st.as<const uint32_t>(), en.as<const uint32_t>(), cp, S.as<double>(),
-                     med.as<uint32_t>(), cnt + 3, heavy.as<uint32_t>(), cnt + 1);
-  OSC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_cell_medium, dim3(4096), dim3(kThreads), 0, c->stream,
-                     xs.as<const double>(), ys.as<const double>(), zs.as<const double>(), qsp,
-                     st.as<const uint32_t>(), en.as<const uint32_t>(), cp, S.as<double>(),
-                     med.as<const uint32_t>(), cnt + 3);
-  OSC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_cell_heavy, dim3(1024), dim3(kThreads), 0, c->stream,
-                     xs.as<const double>(), ys.as<const double>(), zs.as<const double>(), qsp,
-                     st.as<const uint32_t>(), en.as<const uint32_t>(), cp, S.as<double>(),
-                     heavy.as<const uint32_t>(), cnt + 1);
-  OSC_HIP(hipGetLastError());
   ps_cells.stop();
   {
     ProfScope ps(c, ST_DEP_NODES);

@@ -72,7 +72,7 @@ template <typename T>
 struct ZinvIO {
   int64_t sy, sz;         // spectrum strides (KXP, KXP*Ly)
   int64_t o_sy, o_sz, o_so;  // output strides
-  int Ly, Lz, nb, w0, n_out, pad_;
+  int Ly, Lz, nb, w0, n_out, nout;
   OutTerms<T> out[kMaxOut];
 };
 
@@ -82,10 +82,15 @@ struct ZinvIO {
 template <typename T, int L>
 struct Geo {
   static constexpr int TPL = PlanInfo<L>::TPL;
+  // lines per workgroup: at least one 128-B cache line of columns (strided
+  // passes read BX adjacent columns per row), within 1024 threads and 64 KB LDS
   static constexpr int bx_of(int tpl) {
     int bx = 256 / tpl;
+    const int line = 128 / (int)sizeof(cpx<T>);
+    if (bx < line) bx = line;
     if (bx < 1) bx = 1;
-    while (bx > 1 && bx * L * (int)sizeof(cpx<T>) > 49152) bx /= 2;
+    while (bx > 1 && (bx * tpl > 1024 || LdsPad<T>::size(bx * L) * (int)sizeof(cpx<T>) > 65536))
+      bx /= 2;
     return bx;
   }
   static constexpr int BX = PlanInfo<L>::ns ? bx_of(TPL) : 1;
@@ -334,16 +339,21 @@ __global__ void __launch_bounds__((Geo<T, L>::NT))
   using F = FftTile<T, L, 1, BX, false>;
   __shared__ cpx<T> lds[F::LDS];
   F f;
-  const int kx = blockIdx.x * BX + f.b, ky = blockIdx.y, o = blockIdx.z;
+  const int kx = blockIdx.x * BX + f.b, ky = blockIdx.y;
   const bool ok = kx < a.nb;
-  const OutTerms<T>& ot = a.out[o];
-  f.load([&](int, int p) { return ok ? combine(rho, a, ot, kx, ky, p) : mk<T>(0, 0); });
-  f.run(lds, tw);
-  cpx<T>* ob = out + o * a.o_so + kx + ky * a.o_sy;
-  f.store([&](int, int p, cpx<T> v) {
-    const int q = p - a.w0;
-    if (ok && q >= 0 && q < a.n_out) ob[q * a.o_sz] = v;
-  });
+  // every output of this (kx, ky) column in one block: the rho_hat column is
+  // read from HBM once and from L2 for the following outputs
+  for (int o = 0; o < a.nout; ++o) {
+    const OutTerms<T>& ot = a.out[o];
+    f.load([&](int, int p) { return ok ? combine(rho, a, ot, kx, ky, p) : mk<T>(0, 0); });
+    if (o) __syncthreads();  // the previous transform's last LDS reads are done
+    f.run(lds, tw);
+    cpx<T>* ob = out + o * a.o_so + kx + ky * a.o_sy;
+    f.store([&](int, int p, cpx<T> v) {
+      const int q = p - a.w0;
+      if (ok && q >= 0 && q < a.n_out) ob[q * a.o_sz] = v;
+    });
+  }
 }
 
 template <typename T>

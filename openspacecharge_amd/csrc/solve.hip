@@ -10,12 +10,8 @@
 
 namespace osc {
 
-// constants exactly as the upstream code folds them
-constexpr double kFpeiSolver2 = 8987551502.2918644;               // open_spacecharge_mod.f90:547
-constexpr double kFpeiConv3d = 299792458.0 * 299792458.0 * 1e-7;  // core:515,1250
-constexpr double kFpeiPipe = 8987551744.0;                        // core:783 (REAL(4))
-constexpr double kClightSp = 299792448.0;                         // open_spacecharge_mod.f90:436
-constexpr double kClightDp = 299792458.0;                         // core:51,555,929
+// speed of light of the legacy B-field formula (core:51,555,929)
+constexpr double kClightDp = 299792458.0;
 
 namespace {
 constexpr int kThreads = 256;
@@ -177,6 +173,7 @@ void conv_solve(osc_ctx* c, const double* rho, const int n[3], const SpecDims& d
   zi.nb = d.KX;
   zi.w0 = win[2];
   zi.n_out = nz;
+  zi.nout = nout;
   for (int o = 0; o < nout; ++o) {
     zi.out[o].nt = outs[o].nt;
     for (int t = 0; t < outs[o].nt; ++t) {
