@@ -48,6 +48,14 @@ def parse():
     p.add_argument("--cpu-child", nargs=2, metavar=("IN", "OUT"), help=argparse.SUPPRESS)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a 1 GiB device copy (profiler counter calibration)")
+    p.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
+    p.add_argument("--no-traffic", action="store_true",
+                   help="skip the rocprofv3 counter passes (roofline traffic = null)")
+    p.add_argument("--traffic-timeout", type=float, default=120.0)
+    p.add_argument("--batch-bunches", type=int, default=64,
+                   help="bunches of the batched mixed-grid leg (0 skips it)")
+    p.add_argument("--served-requests", type=int, default=64,
+                   help="requests of the served mixed stream (0 skips it)")
     return p.parse_args()
 
 
@@ -166,6 +174,149 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+# ---------------------------------------------------------------------------
+# mixed-size streams: batched free-space bunches and a served request mix
+# ---------------------------------------------------------------------------
+EDGES = sorted(e for e in {2 ** a * 3 ** b for a in range(9) for b in range(6)} if 32 <= e <= 256)
+PIPE_A, PIPE_B = 10e-3, 6e-3
+
+
+def gpu_bunch(n, sigma, seed, center=(0.0, 0.0, 0.0), zpos=False):
+    """Seeded truncated (4 sigma) Gaussian generated on the GPU (torch Philox)."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(int(seed))
+    out = []
+    have = 0
+    while have < n:
+        r = torch.randn((int((n - have) * 1.2) + 64, 3), generator=g, device="cuda",
+                        dtype=torch.float64)
+        r = r[(r * r).sum(1) <= 16.0]
+        r = r * torch.tensor(sigma, dtype=torch.float64, device="cuda") + torch.tensor(
+            center, dtype=torch.float64, device="cuda")
+        if zpos:
+            r = r[r[:, 2] > 0]
+        out.append(r)
+        have += r.shape[0]
+    r = torch.cat(out)[:n]
+    return r[:, 0].contiguous(), r[:, 1].contiguous(), r[:, 2].contiguous()
+
+
+def make_stream(kind, count, seed):
+    """Request list.  kind "batch": free-space bunches with grids from
+    {32^3, 48^3, 64^3, 96x96x128, 128^3}, 1e5-4e6 particles (log-uniform),
+    sigma_z/sigma_x log-uniform in [0.05, 20].  kind "served": 70/20/10 %
+    free/cathode/pipe, grid edge log-uniform over 32..256 snapped to 2^a 3^b,
+    1e5-2e7 particles (log-uniform), varied sigma."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    reqs = []
+    for k in range(count):
+        if kind == "batch":
+            solver = "free"
+            nhi = [(32,) * 3, (48,) * 3, (64,) * 3, (96, 96, 128), (128,) * 3][rng.integers(5)]
+            n = int(np.exp(rng.uniform(np.log(1e5), np.log(4e6))))
+        else:
+            u = rng.random()
+            solver = "free" if u < 0.7 else ("cathode" if u < 0.9 else "pipe")
+            e = np.exp(rng.uniform(np.log(32), np.log(256)))
+            edge = min(EDGES, key=lambda v: abs(np.log(v / e)))
+            nhi = (edge,) * 3
+            n = int(np.exp(rng.uniform(np.log(1e5), np.log(2e7))))
+        ratio = float(np.exp(rng.uniform(np.log(0.05), np.log(20))))
+        sx = 1e-3
+        sig = (sx, sx, sx * ratio)
+        kw = dict(nhi=nhi, gamma=GAMMA, total_charge=QTOT, solver=solver)
+        center = (0.0, 0.0, 0.0)
+        if solver == "cathode":
+            center = (0.0, 0.0, 1.5 * sig[2] + 0.5e-3)
+        if solver == "pipe":
+            sig = (1e-3, 0.6e-3, min(sig[2], 3e-3))
+            kw.update(mesh_min=(-PIPE_A / 2, -PIPE_B / 2, -6 * sig[2]),
+                      mesh_max=(PIPE_A / 2, PIPE_B / 2, 6 * sig[2]), apipe=PIPE_A, bpipe=PIPE_B)
+        reqs.append(dict(n=n, sigma=sig, center=center, seed=seed * 1000 + k,
+                         zpos=solver == "cathode", kw=kw))
+    return reqs
+
+
+def run_stream(reqs, parts, nworkers, precision, green_cache, sequential, rate=None, seed=1):
+    """Runs the requests through a Server; returns (wall s, per-request
+    records, outputs).  sequential: each request waits for the previous one
+    (plain per-request solving).  rate: Poisson arrivals at this many
+    requests/s (otherwise all at once)."""
+    import torch
+    import openspacecharge_amd as osc
+    srv = osc.Server(nworkers=nworkers, green_cache=green_cache)
+    outs, tickets, recs = [], [], []
+    rng = np.random.Generator(np.random.Philox(seed))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t_next = t0
+    for r, (x, y, z) in zip(reqs, parts):
+        if rate:
+            t_next += rng.exponential(1.0 / rate)
+            while time.perf_counter() < t_next:
+                time.sleep(max(0.0, min(1e-3, t_next - time.perf_counter())))
+        E = torch.empty((x.shape[0], 3), dtype=torch.float64, device="cuda")
+        tk = srv.submit(x, y, z, E, precision=precision, **r["kw"])
+        if sequential:
+            recs.append(srv.wait(tk))
+        else:
+            tickets.append(tk)
+        outs.append(E)
+    for tk in tickets:
+        recs.append(srv.wait(tk))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    srv.close()
+    return wall, recs, outs
+
+
+def stream_legs(args):
+    import torch
+    import openspacecharge_amd as osc
+    res = {}
+    for kind, count in (("batch", args.batch_bunches), ("served", args.served_requests)):
+        if count <= 0:
+            continue
+        reqs = make_stream(kind, count, seed=7 if kind == "batch" else 11)
+        parts = [gpu_bunch(r["n"], r["sigma"], r["seed"], r["center"], r["zpos"]) for r in reqs]
+        torch.cuda.synchronize()
+        # warm up (kernel loading), not timed
+        run_stream(reqs[:2], parts[:2], 1, osc.OSC_PREC_FP64, False, True)
+        # plain FP64 per request: one at a time, the Green function rebuilt for every request
+        w64, _, o64 = run_stream(reqs, parts, 1, osc.OSC_PREC_FP64, False, True)
+        # served: concurrent workers, Green spectra kept resident
+        ws64, _, _ = run_stream(reqs, parts, 4, osc.OSC_PREC_FP64, True, False)
+        ws32, recs32, o32 = run_stream(reqs, parts, 4, osc.OSC_PREC_FP32, True, False)
+        err = 0.0
+        for a, b in zip(o32, o64):
+            a, b = a.cpu().numpy(), b.cpu().numpy()
+            err = max(err, max(float(np.max(np.abs(a[:, c] - b[:, c])) / np.max(np.abs(b[:, c])))
+                               for c in range(3)))
+        out = {"requests": count, "particles_total": int(sum(r["n"] for r in reqs)),
+               "fp64_per_request_solves_per_s": round(count / w64, 2),
+               "fp64_served_solves_per_s": round(count / ws64, 2),
+               "fp32_served_solves_per_s": round(count / ws32, 2),
+               "fp32_served_vs_fp64_per_request": round(w64 / ws32, 3),
+               "max_rel_err_fp32_vs_fp64": err}
+        if kind == "served":
+            out["mix"] = "70% free / 20% cathode / 10% pipe, grid edge 32..256 (2^a 3^b), 1e5-2e7 particles"
+            rate = 0.7 * count / ws32
+            wl, recl, _ = run_stream(reqs, parts, 4, osc.OSC_PREC_FP32, True, False, rate=rate)
+            lat = [r["total_ms"] for r in recl]
+            out.update({"poisson_rate_per_s": round(rate, 2),
+                        "fp32_p50_ms": round(float(np.percentile(lat, 50)), 3),
+                        "fp32_p99_ms": round(float(np.percentile(lat, 99)), 3),
+                        "fp32_rate_solves_per_s": round(count / wl, 2)})
+        else:
+            out["mix"] = "free space, grids {32^3,48^3,64^3,96x96x128,128^3}, 1e5-4e6 particles"
+        res[kind] = out
+        log(f"{kind}: {out}")
+        del parts, o64, o32
+        torch.cuda.empty_cache()
+    return res
+
+
 def cpu_leg(args, x, y, z):
     """Runs cpu_child() in a child process started before this process touches
     the GPU, with the stack limit raised (upstream's FFT transposes keep large
@@ -234,6 +385,88 @@ def cpu_child(args):
     print(json.dumps(info))
 
 
+# stage of the per-stage HIP-event timing -> the one kernel it times
+STAGE_KERNEL = {"gather": "k_interp_rec<true>", "fft.inv_zmul": "k_zinv<double",
+                "fft.inv_x": "k_xc2r<double", "fft.fwd_x": "k_xr2c<double"}
+
+
+def traffic_child(args):
+    """A few fused FP64 steps (the workload of the headline), for profiler
+    counter passes."""
+    import torch
+    import openspacecharge_amd as osc
+    x, y, z = bunch(args.particles)
+    dev = torch.device("cuda:0")
+    xs, ys, zs = (torch.from_numpy(a).to(dev) for a in (x, y, z))
+    ctx = osc.Context(0)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    ctx.set_async(True)
+    ctx.set_green_cache(False)
+    mesh = osc.Mesh3D(nhi=(args.grid,) * 3, gamma=GAMMA, device="cuda")
+    for _ in range(2):
+        osc.space_charge_step(xs, ys, zs, mesh, total_charge=QTOT, calc_bfield=True, ctx=ctx)
+    torch.cuda.synchronize()
+    ctx.close()
+
+
+def _counter_csv(d):
+    import glob
+    f = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    return f[0] if f else None
+
+
+def traffic_leg(args):
+    """HBM bytes per launch of each kernel of the fused step: two rocprofv3
+    counter passes (FETCH_SIZE, WRITE_SIZE) over a child running the step,
+    each scaled by the same counter's reading for a 1 GiB device copy.
+    Returns ({kernel name: (read bytes, write bytes)}, scales) or None."""
+    import csv
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    rp = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(rp):
+        return None
+    per = {}
+    scale = {}
+    with tempfile.TemporaryDirectory() as td:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            for what, extra in (("step", ["--traffic-child"]), ("copy", ["--calibrate-copy"])):
+                d = os.path.join(td, f"{ctr}_{what}")
+                cmd = [rp, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "p", "--",
+                       sys.executable, os.path.abspath(__file__), *extra, "--particles",
+                       str(args.particles), "--grid", str(args.grid)]
+                p = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                                     start_new_session=True)
+                try:
+                    p.wait(timeout=args.traffic_timeout)
+                except subprocess.TimeoutExpired:
+                    os.killpg(p.pid, signal.SIGKILL)
+                    p.wait()
+                    log(f"traffic pass {ctr}/{what} timed out")
+                    return None
+                f = _counter_csv(d)
+                if p.returncode != 0 or f is None:
+                    log(f"traffic pass {ctr}/{what} failed (exit {p.returncode})")
+                    return None
+                vals = {}
+                for r in csv.DictReader(open(f)):
+                    vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]) * 1024.0)
+                if what == "copy":
+                    k = [k for k in vals if "copyBuffer" in k]
+                    if not k:
+                        return None
+                    scale[ctr] = (1 << 30) / (sum(vals[k[0]]) / len(vals[k[0]]))
+                else:
+                    for k, v in vals.items():
+                        per.setdefault(k, {})[ctr] = sum(v) / len(v)
+            log(f"traffic pass {ctr} done")
+    out = {k: (v.get("FETCH_SIZE", 0.0) * scale["FETCH_SIZE"], v.get("WRITE_SIZE", 0.0) * scale["WRITE_SIZE"])
+           for k, v in per.items()}
+    return out, scale
+
+
 def calibrate_copy():
     """A plain device copy of known size, for scaling the FETCH_SIZE and
     WRITE_SIZE counters of the same profiler run to bytes."""
@@ -261,6 +494,9 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
+    if args.traffic_child:
+        traffic_child(args)
+        return
     x, y, z = bunch(args.particles)
     log(f"bunch ready: {x.size} particles")
     cpu = e_ref = d_ref = None
@@ -269,8 +505,14 @@ def main():
         t = time.perf_counter()
         cpu, e_ref, d_ref = cpu_leg(args, x, y, z)
         log(f"cpu leg done in {time.perf_counter() - t:.1f} s")
+    traffic = None
+    if not args.no_traffic:
+        t = time.perf_counter()
+        traffic = traffic_leg(args)
+        log(f"traffic passes done in {time.perf_counter() - t:.1f} s")
     res, fields = gpu_leg(args, x, y, z)
     log("gpu legs done")
+    streams = stream_legs(args)
     r64 = res["fp64"]
     # roofline of the dominant memory-bound stage
     n3 = (args.grid,) * 3
@@ -282,6 +524,12 @@ def main():
     per_launch_s = ms_tot / launches / 1e3
     alg = stage_bytes(dom, 8, n3, L, args.particles, 3, 3)
     achieved = alg / per_launch_s / 1e9
+    kname = STAGE_KERNEL.get(dom)
+    traffic_bytes = None
+    if traffic and kname:
+        hits = [v for k, v in traffic[0].items() if kname in k]
+        if hits:
+            traffic_bytes = round(hits[0][0] + hits[0][1])
     out = {
         "metric": METRIC, "value": round(r64["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r64["ms_per_step"], 3),
@@ -303,11 +551,17 @@ def main():
                                       "fp32": round(res["fp32"]["cached_green_solves_per_s"], 3)},
         "three_call_solves_per_s": {"fp64": round(r64["three_call_solves_per_s"], 3),
                                     "fp32": round(res["fp32"]["three_call_solves_per_s"], 3)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "algorithmic_bytes_per_launch": alg,
-                     "avg_launch_ms": round(per_launch_s * 1e3, 5)},
+        "roofline": {"bound": "hbm", "kernel": f"{kname or dom} (stage {dom})",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_bytes,
+                     "algorithmic_bytes_per_launch": alg,
+                     "avg_launch_ms": round(per_launch_s * 1e3, 5),
+                     "traffic_note": "FETCH_SIZE x1024 x %.3f + WRITE_SIZE x1024 x %.3f bytes per launch "
+                                     "(scales from a 1 GiB device copy)" % (
+                                         traffic[1]["FETCH_SIZE"] if traffic else float("nan"),
+                                         traffic[1]["WRITE_SIZE"] if traffic else float("nan"))},
     }
+    out.update(streams)
     if cpu is not None:
         out["cpu_baseline"] = cpu
         # full-size FP64 parity against the upstream library on the same bunch

@@ -89,7 +89,7 @@ int osc_abi_version(void);
  * bracketed by events of its stage; osc_ctx_profile_read synchronises, adds
  * the elapsed times (ms) and launch counts of each stage into the arrays
  * (OSC_NSTAGE entries) and clears the records. */
-#define OSC_NSTAGE 16
+#define OSC_NSTAGE 18
 int osc_ctx_set_profiling(osc_ctx* ctx, int on);
 int osc_ctx_profile_read(osc_ctx* ctx, double* ms, int64_t* launches);
 const char* osc_stage_name(int stage);
@@ -218,6 +218,44 @@ int osc_interpolate_field(osc_ctx* ctx, const osc_mesh* mesh, const double* efie
                           const double* bfield, int64_t n, const double* x, const double* y,
                           const double* z, double* E, double* B, int8_t* clamped,
                           int64_t* n_clamped);
+
+/* ---- request loop (no upstream counterpart) -----------------------------------
+ * A stream of independent solves -- free space, cathode images (the
+ * space_charge_3d at_cathode path) and rectangular pipe -- each one
+ * deposit -> solve -> gather of E (and B) at its particles, as a tracking code
+ * would call deposit_particles, the solver and interpolate_field.  nworkers
+ * host threads each own a context and HIP stream, so requests overlap on the
+ * GPU; requests with the same solver, mesh shape, fixed mesh geometry, gamma
+ * and pipe size go to the same worker and reuse its cached Green spectra
+ * (green_cache != 0).  A request starts when its estimated device bytes fit
+ * the budget next to the running ones (max_bytes, or 7/8 of the free memory
+ * when max_bytes <= 0).  Particle and output arrays are device memory and
+ * must stay valid until osc_server_wait returns for the request. */
+enum osc_solver { OSC_SOLVE_FREE = 0, OSC_SOLVE_CATHODE = 1, OSC_SOLVE_PIPE = 2 };
+typedef struct osc_request {
+  int32_t solver;          /* osc_solver                                             */
+  int32_t precision;       /* OSC_PREC_FP64 or OSC_PREC_FP32                          */
+  int32_t nhi[3];          /* mesh nodes per axis (nlo = npad = 1)                    */
+  int32_t resize_mesh;     /* 1: mesh from the bunch (deposit_particles rule); 0: min/max */
+  int64_t n;               /* particles                                              */
+  const double *x, *y, *z; /* coordinates                                            */
+  double total_charge;
+  double gamma;
+  double min[3], max[3];   /* fixed mesh when resize_mesh == 0 (required by the pipe)  */
+  double apipe, bpipe;     /* pipe cross-section (OSC_SOLVE_PIPE)                     */
+  double *E, *B;           /* (3,n) outputs; B may be NULL                             */
+} osc_request;
+typedef struct osc_server osc_server;
+int osc_server_create(int device, int nworkers, int64_t max_bytes, int green_cache,
+                      osc_server** server);
+int osc_server_submit(osc_server* server, const osc_request* req, int64_t* ticket);
+/* blocks until the request is done; times_ms (3, may be NULL): queued, run,
+ * total (submit to done); counts (2, may be NULL): particles dropped by the
+ * deposit, points clamped by the gather.  Returns the request's status. */
+int osc_server_wait(osc_server* server, int64_t ticket, double* times_ms, int64_t* counts);
+int osc_server_drain(osc_server* server);
+const char* osc_server_last_error(const osc_server* server);
+int osc_server_destroy(osc_server* server);
 
 #ifdef __cplusplus
 }

@@ -6,6 +6,7 @@ include/opensc_hip.h).  There is no CPU fallback: without the built library
 every call raises ``LibraryNotBuilt``.
 """
 from ._capi import (OSC_PREC_FP32, OSC_PREC_FP64, Context, LibraryNotBuilt, OSCError)  # noqa: F401
+from .api import Server  # noqa: F401
 from .api import (Mesh3D, OpenSCStop, ccfft3d, context, deposit_particles,  # noqa: F401
                   depose_rho_scalar, get_mesh_quantities, interpolate_field, osc_alloc_rectpipe_arrays,
                   osc_freespace_solver2, osc_get_cgrn_freespace, osc_getgrnfree,

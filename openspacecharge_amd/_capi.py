@@ -40,6 +40,33 @@ class OSCError(RuntimeError):
         self.msg = msg
 
 
+OSC_SOLVE_FREE = 0
+OSC_SOLVE_CATHODE = 1
+OSC_SOLVE_PIPE = 2
+
+
+class OscRequest(C.Structure):
+    """osc_request (one solve of the request loop)."""
+    _fields_ = [
+        ("solver", C.c_int32),
+        ("precision", C.c_int32),
+        ("nhi", C.c_int32 * 3),
+        ("resize_mesh", C.c_int32),
+        ("n", C.c_int64),
+        ("x", C.c_void_p),
+        ("y", C.c_void_p),
+        ("z", C.c_void_p),
+        ("total_charge", C.c_double),
+        ("gamma", C.c_double),
+        ("min", C.c_double * 3),
+        ("max", C.c_double * 3),
+        ("apipe", C.c_double),
+        ("bpipe", C.c_double),
+        ("E", C.c_void_p),
+        ("B", C.c_void_p),
+    ]
+
+
 class OscMesh(C.Structure):
     """osc_mesh (mesh3d_struct without arrays)."""
     _fields_ = [
@@ -110,6 +137,12 @@ SIGNATURES = {
                                             C.c_int, _vp, _vp, _vp, _vp]),
     "osc_interpolate_field": (C.c_int, [_ctxp, _meshp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp,
                                         _vp, C.POINTER(_i64)]),
+    "osc_server_create": (C.c_int, [C.c_int, C.c_int, _i64, C.c_int, C.POINTER(_vp)]),
+    "osc_server_submit": (C.c_int, [_vp, C.c_void_p, C.POINTER(_i64)]),
+    "osc_server_wait": (C.c_int, [_vp, _i64, _dp, C.POINTER(_i64)]),
+    "osc_server_drain": (C.c_int, [_vp]),
+    "osc_server_last_error": (C.c_char_p, [_vp]),
+    "osc_server_destroy": (C.c_int, [_vp]),
 }
 
 _lib = None
@@ -184,7 +217,7 @@ class Context:
     def synchronize(self):
         self.check(self.lib.osc_ctx_synchronize(self.h))
 
-    NSTAGE = 16
+    NSTAGE = 18
 
     def set_profiling(self, on: bool):
         self.check(self.lib.osc_ctx_set_profiling(self.h, int(bool(on))))

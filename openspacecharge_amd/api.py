@@ -510,3 +510,89 @@ def interpolate_field(x, y, z, mesh3d: Mesh3D, E=True, B=False, ctx: Optional[K.
     if return_clamped:
         return res, cl
     return res
+
+
+# ---------------------------------------------------------------------------
+# request loop (osc_server_*): independent solves overlapping on one GPU
+# ---------------------------------------------------------------------------
+_SOLVERS = {"free": K.OSC_SOLVE_FREE, "cathode": K.OSC_SOLVE_CATHODE, "pipe": K.OSC_SOLVE_PIPE}
+
+
+class Server:
+    """A stream of solves (free space, cathode images, rectangular pipe), each
+    deposit -> solve -> gather at its particles.  ``nworkers`` contexts on
+    their own HIP streams run requests concurrently; requests with the same
+    geometry share a worker and its cached Green spectra; a request starts
+    when its estimated device bytes fit the budget next to the running ones.
+    Particle and output arrays are torch tensors on the GPU and are kept
+    alive by the server until ``wait`` returns for the request."""
+
+    def __init__(self, nworkers: int = 4, device: int = 0, max_bytes: int = 0,
+                 green_cache: bool = True):
+        self.lib = K.load()
+        h = C.c_void_p()
+        rc = self.lib.osc_server_create(int(device), int(nworkers), int(max_bytes),
+                                        int(bool(green_cache)), C.byref(h))
+        if rc != K.OSC_OK:
+            raise K.OSCError(rc, "osc_server_create failed (no usable GPU?)")
+        self.h = h
+        self._keep = {}
+
+    def submit(self, x, y, z, E, B=None, *, nhi, gamma, total_charge, solver="free",
+               precision=K.OSC_PREC_FP64, mesh_min=None, mesh_max=None, apipe=0.0,
+               bpipe=0.0) -> int:
+        """Queues one request; E (and B) are (n, 3) float64 tensors receiving
+        the fields at the particles.  mesh_min/mesh_max give a fixed mesh
+        (required for the pipe); otherwise the mesh follows the bunch."""
+        for a in (x, y, z, E) + ((B,) if B is not None else ()):
+            if not (_is_torch(a) and a.is_cuda and a.is_contiguous()):
+                raise ValueError("server requests take contiguous CUDA tensors")
+        r = K.OscRequest()
+        r.solver = _SOLVERS[solver]
+        r.precision = int(precision)
+        for i in range(3):
+            r.nhi[i] = int(nhi[i])
+        r.resize_mesh = 0 if mesh_min is not None else 1
+        r.n = int(x.shape[0])
+        r.x, r.y, r.z = x.data_ptr(), y.data_ptr(), z.data_ptr()
+        r.total_charge = float(total_charge)
+        r.gamma = float(gamma)
+        if mesh_min is not None:
+            for i in range(3):
+                r.min[i], r.max[i] = float(mesh_min[i]), float(mesh_max[i])
+        r.apipe, r.bpipe = float(apipe), float(bpipe)
+        r.E = E.data_ptr()
+        r.B = B.data_ptr() if B is not None else None
+        t = C.c_int64(0)
+        rc = self.lib.osc_server_submit(self.h, C.byref(r), C.byref(t))
+        if rc != K.OSC_OK:
+            raise K.OSCError(rc, "osc_server_submit: bad request")
+        self._keep[t.value] = (x, y, z, E, B)
+        return t.value
+
+    def wait(self, ticket: int) -> dict:
+        """Blocks until the request is done; returns its times (ms) and counts."""
+        tm = (C.c_double * 3)()
+        cn = (C.c_int64 * 2)()
+        rc = self.lib.osc_server_wait(self.h, int(ticket), tm, cn)
+        self._keep.pop(ticket, None)
+        if rc != K.OSC_OK:
+            msg = self.lib.osc_server_last_error(self.h)
+            raise K.OSCError(rc, msg.decode() if msg else "")
+        return dict(queued_ms=tm[0], run_ms=tm[1], total_ms=tm[2], n_dropped=cn[0],
+                    n_clamped=cn[1])
+
+    def drain(self) -> None:
+        self.lib.osc_server_drain(self.h)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.osc_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

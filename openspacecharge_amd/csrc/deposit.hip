@@ -23,14 +23,34 @@ inline int64_t nblk(int64_t n) {
 // ---------------------------------------------------------------------------
 // reductions (fixed partition and tree: deterministic)
 // ---------------------------------------------------------------------------
-constexpr int kRedBlocks = 1024;
+constexpr int kRedBlocks = 2048;
 
 __global__ void k_bbox_partial(const double* __restrict__ x, const double* __restrict__ y,
                                const double* __restrict__ z, int64_t n, double* __restrict__ part) {
   __shared__ double r[6][kThreads];
   double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * kThreads) {
+  const int64_t stride = int64_t(gridDim.x) * kThreads;
+  int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x;
+  // four independent loads of each coordinate in flight per thread
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    double a[4], b[4], c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = x[i + u * stride];
+      b[u] = y[i + u * stride];
+      c[u] = z[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      v[0] = fmin(v[0], a[u]);
+      v[1] = fmin(v[1], b[u]);
+      v[2] = fmin(v[2], c[u]);
+      v[3] = fmax(v[3], a[u]);
+      v[4] = fmax(v[4], b[u]);
+      v[5] = fmax(v[5], c[u]);
+    }
+  }
+  for (; i < n; i += stride) {
     v[0] = fmin(v[0], x[i]);
     v[1] = fmin(v[1], y[i]);
     v[2] = fmin(v[2], z[i]);
@@ -101,8 +121,9 @@ __global__ void k_sum_final(const double* __restrict__ part, int nb, double* __r
 // ---------------------------------------------------------------------------
 struct KeyParams {
   double mn[3], dinv[3], base[3];
-  int nlo[3], nn[3];
+  int nlo[3], nn[3], nhi[3];
   uint32_t sentinel;
+  int clamp;  // 0: deposit rule (off-mesh particles get the sentinel); 1: gather rule
 };
 
 __device__ __forceinline__ bool cell_of(double x, double mn, double dinv, double base, int nlo,
@@ -115,21 +136,44 @@ __device__ __forceinline__ bool cell_of(double x, double mn, double dinv, double
   return true;
 }
 
+// cell read by the gather: interpolate_field's clamp to [1, nhi-1] (:320-355),
+// then the array cell inside a field with lower bound nlo
+__device__ __forceinline__ int gather_cell(double x, double mn, double dinv, int nlo, int nhi,
+                                           int nn) {
+  const double f = floor((x - mn) * dinv + 1.0);
+  const int ip = !(f >= 1.0) ? 1 : (f > double(nhi - 1) ? nhi - 1 : int(f));
+  const int i0 = ip - nlo;
+  return i0 < 0 ? 0 : (i0 > nn - 2 ? nn - 2 : i0);
+}
+
+// cell key of every particle, and its 32-byte record (x, y, z, q) in input
+// order (the sort's payload: one record per particle instead of three
+// scattered coordinates)
 __global__ void k_cic_keys(const double* __restrict__ x, const double* __restrict__ y,
-                           const double* __restrict__ z, int64_t n, KeyParams p,
-                           uint32_t* __restrict__ keys, uint32_t* __restrict__ ndrop) {
+                           const double* __restrict__ z, const double* __restrict__ q, int64_t n,
+                           KeyParams p, uint32_t* __restrict__ keys, double4* __restrict__ rec,
+                           uint32_t* __restrict__ ndrop) {
   __shared__ uint32_t cnt;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   uint32_t mine = 0;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
+    const double xi = x[i], yi = y[i], zi = z[i];
     int i0, j0, k0;
-    bool ok = cell_of(x[i], p.mn[0], p.dinv[0], p.base[0], p.nlo[0], p.nn[0], i0) &&
-              cell_of(y[i], p.mn[1], p.dinv[1], p.base[1], p.nlo[1], p.nn[1], j0) &&
-              cell_of(z[i], p.mn[2], p.dinv[2], p.base[2], p.nlo[2], p.nn[2], k0);
+    bool ok = true;
+    if (p.clamp) {
+      i0 = gather_cell(xi, p.mn[0], p.dinv[0], p.nlo[0], p.nhi[0], p.nn[0]);
+      j0 = gather_cell(yi, p.mn[1], p.dinv[1], p.nlo[1], p.nhi[1], p.nn[1]);
+      k0 = gather_cell(zi, p.mn[2], p.dinv[2], p.nlo[2], p.nhi[2], p.nn[2]);
+    } else {
+      ok = cell_of(xi, p.mn[0], p.dinv[0], p.base[0], p.nlo[0], p.nn[0], i0) &&
+           cell_of(yi, p.mn[1], p.dinv[1], p.base[1], p.nlo[1], p.nn[1], j0) &&
+           cell_of(zi, p.mn[2], p.dinv[2], p.base[2], p.nlo[2], p.nn[2], k0);
+    }
     keys[i] = ok ? uint32_t(i0) + uint32_t(p.nn[0]) * (uint32_t(j0) + uint32_t(p.nn[1]) * uint32_t(k0))
                  : p.sentinel;
+    rec[i] = make_double4(xi, yi, zi, q ? q[i] : 0.0);
     mine += ok ? 0u : 1u;
   }
   if (mine) atomicAdd(&cnt, mine);
@@ -157,10 +201,10 @@ __global__ void __launch_bounds__(RS_T)
   for (int d = threadIdx.x; d <= int(mask); d += RS_T) hist[int64_t(d) * nb + blockIdx.x] = h[d];
 }
 
-// payload gathered by the last pass: coordinates (and charges) in sorted order
+// payload gathered by the last pass: the particle records in sorted order
 struct SortPayload {
-  const double *x, *y, *z, *q;
-  double *xs, *ys, *zs, *qs;
+  const double4* rec;
+  double4* srec;
 };
 
 __global__ void __launch_bounds__(RS_T)
@@ -197,12 +241,7 @@ __global__ void __launch_bounds__(RS_T)
       for (int w = 0; w < wave; ++w) pos += wc[w][d];
       kout[pos] = k;
       vout[pos] = v;
-      if (pl.xs) {
-        pl.xs[pos] = pl.x[v];
-        pl.ys[pos] = pl.y[v];
-        pl.zs[pos] = pl.z[v];
-        if (pl.qs) pl.qs[pos] = pl.q[v];
-      }
+      if (pl.srec) pl.srec[pos] = pl.rec[v];
     }
     __syncthreads();
     for (int dd = threadIdx.x; dd <= int(mask); dd += RS_T) {
@@ -318,9 +357,8 @@ __device__ __forceinline__ void cic_terms(double x, double y, double z, double q
 constexpr int kSegSerial = 64;
 
 __global__ void __launch_bounds__(kThreads)
-    k_seg_reduce(const uint32_t* __restrict__ ks, const double* __restrict__ xs,
-                 const double* __restrict__ ys, const double* __restrict__ zs,
-                 const double* __restrict__ qs, int64_t n, const uint32_t* __restrict__ ndrop,
+    k_seg_reduce(const uint32_t* __restrict__ ks, const double4* __restrict__ srec, int64_t n,
+                 const uint32_t* __restrict__ ndrop,
                  CellParams p, double* __restrict__ S, double* __restrict__ head,
                  double* __restrict__ tail) {
   const int64_t nv = n - int64_t(*ndrop);
@@ -338,7 +376,8 @@ __global__ void __launch_bounds__(kThreads)
       const int i0 = int(k % uint32_t(p.nn[0]));
       const int j0 = int((k / uint32_t(p.nn[0])) % uint32_t(p.nn[1]));
       const int k0 = int(k / (uint32_t(p.nn[0]) * uint32_t(p.nn[1])));
-      cic_terms(xs[i], ys[i], zs[i], p.hasq ? qs[i] : p.q_each, p, i0, j0, k0, t);
+      const double4 r = srec[i];
+      cic_terms(r.x, r.y, r.z, p.hasq ? r.w : p.q_each, p, i0, j0, k0, t);
     }
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -583,9 +622,7 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   const int64_t n = a.n;
   DevBuf ka(c->pool, sizeof(uint32_t) * (n + 1)), va(c->pool, sizeof(uint32_t) * (n + 1));
   DevBuf kb(c->pool, sizeof(uint32_t) * (n + 1)), vb(c->pool, sizeof(uint32_t) * (n + 1));
-  DevBuf xs(c->pool, sizeof(double) * (n + 1)), ys(c->pool, sizeof(double) * (n + 1)),
-      zs(c->pool, sizeof(double) * (n + 1));
-  DevBuf qs(c->pool, a.q ? sizeof(double) * (n + 1) : 0);
+  DevBuf rec(c->pool, sizeof(double4) * (n + 1)), srec(c->pool, sizeof(double4) * (n + 1));
   KeyParams kp{};
   CellParams cp{};
   for (int d = 0; d < 3; ++d) {
@@ -606,15 +643,14 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   cp.q_each = a.q_each;
   if (n > 0) {
     ProfScope ps(c, ST_DEP_KEYS);
-    hipLaunchKernelGGL(k_cic_keys, dim3(nblk(n)), dim3(kThreads), 0, c->stream, a.x, a.y, a.z, n,
-                       kp, ka.as<uint32_t>(), cnt);
+    hipLaunchKernelGGL(k_cic_keys, dim3(nblk(n)), dim3(kThreads), 0, c->stream, a.x, a.y, a.z,
+                       a.q, n, kp, ka.as<uint32_t>(), rec.as<double4>(), cnt);
     OSC_HIP(hipGetLastError());
   }
   int nbits = 0;
   while ((uint64_t(1) << nbits) <= uint64_t(ncell)) ++nbits;
   bool in_b = false;
-  SortPayload pl{a.x, a.y, a.z, a.q, xs.as<double>(), ys.as<double>(), zs.as<double>(),
-                 a.q ? qs.as<double>() : nullptr};
+  SortPayload pl{rec.as<const double4>(), srec.as<double4>()};
   {
     ProfScope ps(c, ST_DEP_SORT);
     radix_sort(c, ka, va, kb, vb, n, nbits, pl, in_b);
@@ -627,11 +663,10 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   DevBuf S(c->pool, sizeof(double) * 8 * ncell), head(c->pool, sizeof(double) * 8 * nw),
       tail(c->pool, sizeof(double) * 8 * nw), heavy(c->pool, sizeof(uint32_t) * nw);
   fill_f64(c, S.as<double>(), 8 * ncell, 0.0);
-  const double* qsp = a.q ? qs.as<const double>() : nullptr;
   if (n > 0) {
     hipLaunchKernelGGL(k_seg_reduce, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk,
-                       xs.as<const double>(), ys.as<const double>(), zs.as<const double>(), qsp, n,
-                       cnt, cp, S.as<double>(), head.as<double>(), tail.as<double>());
+                       srec.as<const double4>(), n, cnt, cp, S.as<double>(), head.as<double>(),
+                       tail.as<double>());
     OSC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_seg_fix, dim3(nblk(nw)), dim3(kThreads), 0, c->stream, sk, n, cnt, ncell,
                        head.as<const double>(), tail.as<const double>(), S.as<double>(),
@@ -652,10 +687,44 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   if (keep) {
     keep->n = n;
     keep->sv = std::move(in_b ? vb : va);
-    keep->xs = std::move(xs);
-    keep->ys = std::move(ys);
-    keep->zs = std::move(zs);
+    keep->srec = std::move(srec);
   }
+}
+
+void sort_for_gather(osc_ctx* c, const InterpArgs& a, int64_t n, const double* x, const double* y,
+                     const double* z, SortedParticles* out) {
+  const int64_t ncell = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
+  OSC_REQUIRE(ncell < (int64_t(1) << 31) && n < (int64_t(1) << 32) - 1, OSC_ERR_ARG,
+              "grid or point count too large for 32-bit keys");
+  DevBuf ka(c->pool, sizeof(uint32_t) * (n + 1)), va(c->pool, sizeof(uint32_t) * (n + 1));
+  DevBuf kb(c->pool, sizeof(uint32_t) * (n + 1)), vb(c->pool, sizeof(uint32_t) * (n + 1));
+  DevBuf rec(c->pool, sizeof(double4) * (n + 1)), srec(c->pool, sizeof(double4) * (n + 1));
+  KeyParams kp{};
+  for (int d = 0; d < 3; ++d) {
+    kp.mn[d] = a.mn[d];
+    kp.dinv[d] = 1.0 / a.delta[d];
+    kp.nlo[d] = a.nlo[d];
+    kp.nhi[d] = a.nhi[d];
+    kp.nn[d] = a.nn[d];
+  }
+  kp.clamp = 1;
+  kp.sentinel = uint32_t(ncell);
+  uint32_t* cnt = static_cast<uint32_t*>(c->counters) + 3;  // unused drop count of the clamp rule
+  fill_u32(c, cnt, 1, 0u);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_cic_keys, dim3(nblk(n)), dim3(kThreads), 0, c->stream, x, y, z,
+                       static_cast<const double*>(nullptr), n, kp, ka.as<uint32_t>(),
+                       rec.as<double4>(), cnt);
+    OSC_HIP(hipGetLastError());
+  }
+  int nbits = 0;
+  while ((uint64_t(1) << nbits) <= uint64_t(ncell)) ++nbits;
+  bool in_b = false;
+  SortPayload pl{rec.as<const double4>(), srec.as<double4>()};
+  radix_sort(c, ka, va, kb, vb, n, nbits, pl, in_b);
+  out->n = n;
+  out->sv = std::move(in_b ? vb : va);
+  out->srec = std::move(srec);
 }
 
 }  // namespace osc

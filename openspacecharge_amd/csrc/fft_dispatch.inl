@@ -40,8 +40,7 @@ template <int L>
 void zinv_t(const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, int Ly, int nout, const cpx<T>* tw,
             hipStream_t s) {
   constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
-  (void)nout;  // the kernel loops over a.nout outputs per block
-  dim3 g = grid3(cdiv(a.nb, BX), Ly, 1);
+  dim3 g = grid3(cdiv(a.nb, BX), Ly, nout);
   hipLaunchKernelGGL((k_zinv<T, L>), g, dim3(NT), 0, s, rho, out, a, tw);
 }
 }  // namespace

@@ -6,6 +6,8 @@ namespace osc {
 
 namespace {
 constexpr int kThreads = 256;
+// batches at least this large are sorted by cell before the gather
+constexpr int64_t kSortMin = 1 << 16;
 
 struct IP {
   double mn[3], delta[3], hinv[3];
@@ -109,13 +111,15 @@ __device__ __forceinline__ double wsum8(const double* __restrict__ R, int64_t sy
          R[o + i1 + j1] * (1. - ab) * (1. - de) * gh + R[o + i1] * (1. - ab) * de * gh;
 }
 
-// SORTED: slot s holds particle sv[s] with coordinates x[s], y[s], z[s]
+// SORTED: slot s holds particle sv[s] with record srec[s] = (x, y, z, q);
+// otherwise slot s is particle s with coordinates x[s], y[s], z[s]
 template <bool SORTED>
 __global__ void k_interp_rec(const double* __restrict__ R, int wantE, int wantB, int64_t n,
                              const double* __restrict__ x, const double* __restrict__ y,
-                             const double* __restrict__ z, const uint32_t* __restrict__ sv, IP p,
-                             double* __restrict__ E, double* __restrict__ B,
-                             int8_t* __restrict__ clamped, uint32_t* __restrict__ nclamp) {
+                             const double* __restrict__ z, const double4* __restrict__ srec,
+                             const uint32_t* __restrict__ sv, IP p, double* __restrict__ E,
+                             double* __restrict__ B, int8_t* __restrict__ clamped,
+                             uint32_t* __restrict__ nclamp) {
   __shared__ uint32_t cnt;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
@@ -123,8 +127,20 @@ __global__ void k_interp_rec(const double* __restrict__ R, int wantE, int wantB,
   const int64_t sy = p.nn[0], sz = int64_t(p.nn[0]) * p.nn[1];
   for (int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; s < n;
        s += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t i = SORTED ? int64_t(sv[s]) : s;
-    const double xi = x[s], yi = y[s], zi = z[s];
+    int64_t i;
+    double xi, yi, zi;
+    if (SORTED) {
+      i = int64_t(sv[s]);
+      const double4 r = srec[s];
+      xi = r.x;
+      yi = r.y;
+      zi = r.z;
+    } else {
+      i = s;
+      xi = x[s];
+      yi = y[s];
+      zi = z[s];
+    }
     bool fl = false;
     const int ip = clamp_idx(xi, p.mn[0], p.hinv[0], p.nhi[0], fl);
     const int jp = clamp_idx(yi, p.mn[1], p.hinv[1], p.nhi[1], fl);
@@ -166,19 +182,23 @@ void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const
   }
   const int64_t nt = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
   uint32_t* cnt = static_cast<uint32_t*>(c->counters) + 2;
-  ProfScope ps(c, ST_GATHER);
   fill_u32(c, cnt, 1, 0u);
   DevBuf rec(c->pool, sizeof(double) * 8 * nt);
   int64_t nb = std::min<int64_t>((nt + kThreads - 1) / kThreads, 65536 * 8);
-  hipLaunchKernelGGL(k_pack_nodes, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream, ef, bf, nt,
-                     rec.as<double>());
-  OSC_HIP(hipGetLastError());
+  {
+    ProfScope pp(c, ST_GATHER_PACK);
+    hipLaunchKernelGGL(k_pack_nodes, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream, ef, bf, nt,
+                       rec.as<double>());
+    OSC_HIP(hipGetLastError());
+  }
+  ProfScope ps(c, ST_GATHER);
   const int64_t n = sp.n;
   if (n > 0) {
     nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
     hipLaunchKernelGGL(k_interp_rec<true>, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
                        rec.as<const double>(), int(E && ef), int(B && bf), n,
-                       sp.xs.as<const double>(), sp.ys.as<const double>(), sp.zs.as<const double>(),
+                       static_cast<const double*>(nullptr), static_cast<const double*>(nullptr),
+                       static_cast<const double*>(nullptr), sp.srec.as<const double4>(),
                        sp.sv.as<const uint32_t>(), p, E, B, clamped, cnt);
     OSC_HIP(hipGetLastError());
   }
@@ -198,10 +218,22 @@ int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const dou
     OSC_REQUIRE(a.nn[d] >= 2, OSC_ERR_ARG, "interpolation needs at least 2 nodes per axis");
   }
   uint32_t* cnt = static_cast<uint32_t*>(c->counters) + 2;
+  const int64_t nt = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
+  if (n >= kSortMin && n >= nt / 8) {
+    // many points in arbitrary order: sort them by cell first, so that the
+    // node records are read with locality (random order reads every node
+    // record line many times over)
+    SortedParticles sp;
+    {
+      ProfScope ps(c, ST_GATHER_SORT);
+      sort_for_gather(c, a, n, x, y, z, &sp);
+    }
+    interpolate_sorted(c, a, ef, bf, sp, E, B, clamped);
+    return 0;
+  }
   ProfScope ps(c, ST_GATHER);
   fill_u32(c, cnt, 1, 0u);
   if (n <= 0) return 0;
-  const int64_t nt = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
   if (n < nt / 8) {
     // few points: read the planar fields directly instead of packing the mesh
     int64_t nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
@@ -212,13 +244,19 @@ int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const dou
   }
   DevBuf rec(c->pool, sizeof(double) * 8 * nt);
   int64_t nb = std::min<int64_t>((nt + kThreads - 1) / kThreads, 65536 * 8);
-  hipLaunchKernelGGL(k_pack_nodes, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream, ef, bf, nt,
-                     rec.as<double>());
-  OSC_HIP(hipGetLastError());
+  ps.stop();
+  {
+    ProfScope pp(c, ST_GATHER_PACK);
+    hipLaunchKernelGGL(k_pack_nodes, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream, ef, bf, nt,
+                       rec.as<double>());
+    OSC_HIP(hipGetLastError());
+  }
+  ProfScope pg(c, ST_GATHER);
   nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
   hipLaunchKernelGGL(k_interp_rec<false>, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
-                     rec.as<const double>(), int(E && ef), int(B && bf), n, x, y, z, nullptr, p, E,
-                     B, clamped, cnt);
+                     rec.as<const double>(), int(E && ef), int(B && bf), n, x, y, z,
+                     static_cast<const double4*>(nullptr), static_cast<const uint32_t*>(nullptr), p,
+                     E, B, clamped, cnt);
   OSC_HIP(hipGetLastError());
   return 0;
 }

@@ -15,53 +15,55 @@ inline int64_t nblk(int64_t n, int t = kThreads) {
 }
 
 // primitive of osc_get_cgrn_freespace on the offset grid
-// (open_spacecharge_mod.f90:674-691): F = prim(u,v,w) * factor
-__global__ void k_prim_offset(double* __restrict__ F, int L0, int L1, int L2, double dx, double dy,
-                              double dz, double umin, double vmin, double wmin, double factor,
-                              int icomp) {
-  const int64_t N = int64_t(L0) * L1 * L2;
-  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < N;
-       idx += int64_t(gridDim.x) * blockDim.x) {
-    const int i = int(idx % L0);
-    const int j = int((idx / L0) % L1);
-    const int k = int(idx / (int64_t(L0) * L1));
-    const double w = double(k) * dz + wmin;
-    const double v = double(j) * dy + vmin;
-    const double u = double(i) * dx + umin;
-    double g;
-    if (icomp == 0) g = lafun2(u, v, w) * factor;
-    else if (icomp == 1) g = xlafun2(u, v, w) * factor;
-    else if (icomp == 2) g = xlafun2(v, w, u) * factor;
-    else g = xlafun2(w, u, v) * factor;
-    F[idx] = g;
-  }
+// (open_spacecharge_mod.f90:674-691): F = prim(u,v,w) * factor.
+// Grid (ceil(L0/64), L1, L2) of 64-thread blocks: no index division.
+// mir >= 0: the primitive is even in that axis (xlafun2 in its first
+// argument: x atan(yz/(rx)) is even in x, r depends on x^2) and the axis has no
+// offset, so its coordinate is taken as (i + 1/2 - L/2) * d -- a half-integer
+// times d, rounded once, so that the mirror point L-1-i is exactly its
+// negative -- and only the lower half along it is evaluated; each value is
+// stored at i and L-1-i.
+__global__ void __launch_bounds__(64)
+    k_prim_offset(double* __restrict__ F, int L0, int L1, int L2, double dx, double dy, double dz,
+                  double umin, double vmin, double wmin, double factor, int icomp, int mir) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  const int lim0 = mir == 0 ? L0 / 2 : L0;
+  if (i >= lim0) return;
+  const double w = mir == 2 ? (double(k) + (0.5 - double(L2 / 2))) * dz : double(k) * dz + wmin;
+  const double v = mir == 1 ? (double(j) + (0.5 - double(L1 / 2))) * dy : double(j) * dy + vmin;
+  const double u = mir == 0 ? (double(i) + (0.5 - double(L0 / 2))) * dx : double(i) * dx + umin;
+  double g;
+  if (icomp == 0) g = lafun2(u, v, w) * factor;
+  else if (icomp == 1) g = xlafun2(u, v, w) * factor;
+  else if (icomp == 2) g = xlafun2(v, w, u) * factor;
+  else g = xlafun2(w, u, v) * factor;
+  F[i + int64_t(L0) * (j + int64_t(L1) * k)] = g;
+  if (mir == 0) F[(L0 - 1 - i) + int64_t(L0) * (j + int64_t(L1) * k)] = g;
+  if (mir == 1) F[i + int64_t(L0) * ((L1 - 1 - j) + int64_t(L1) * k)] = g;
+  if (mir == 2) F[i + int64_t(L0) * (j + int64_t(L1) * (L2 - 1 - k))] = g;
 }
 
 // 8-corner difference over 1:size-1 with the last planes left raw
 // (open_spacecharge_mod.f90:697-705); output z index circularly shifted by sh
-__global__ void k_diff_offset(const double* __restrict__ F, double* __restrict__ G, int L0, int L1,
-                              int L2, int sh, int zero_raw) {
-  const int64_t N = int64_t(L0) * L1 * L2;
+__global__ void __launch_bounds__(64)
+    k_diff_offset(const double* __restrict__ F, double* __restrict__ G, int L0, int L1, int L2,
+                  int sh, int zero_raw) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y, kq = blockIdx.z;
+  if (i >= L0) return;
   const int64_t P = int64_t(L0) * L1;
-  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < N;
-       idx += int64_t(gridDim.x) * blockDim.x) {
-    const int i = int(idx % L0);
-    const int j = int((idx / L0) % L1);
-    const int kq = int(idx / P);
-    int k = (kq - sh) % L2;
-    if (k < 0) k += L2;
-    const int64_t b = i + int64_t(L0) * j + P * k;
-    double g;
-    if (i < L0 - 1 && j < L1 - 1 && k < L2 - 1) {
-      g = F[b + 1 + L0 + P] - F[b + L0 + P] - F[b + 1 + P] - F[b + 1 + L0] - F[b] + F[b + P] +
-          F[b + L0] + F[b + 1];
-    } else {
-      // upstream keeps the raw primitive here; it never reaches the
-      // extracted octant, and zeroing it removes its round-off from the FFT
-      g = zero_raw ? 0.0 : F[b];
-    }
-    G[idx] = g;
+  int k = (kq - sh) % L2;
+  if (k < 0) k += L2;
+  const int64_t b = i + int64_t(L0) * j + P * k;
+  double g;
+  if (i < L0 - 1 && j < L1 - 1 && k < L2 - 1) {
+    g = F[b + 1 + L0 + P] - F[b + L0 + P] - F[b + 1 + P] - F[b + 1 + L0] - F[b] + F[b + P] +
+        F[b + L0] + F[b + 1];
+  } else {
+    // upstream keeps the raw primitive here; it never reaches the
+    // extracted octant, and zeroing it removes its round-off from the FFT
+    g = zero_raw ? 0.0 : F[b];
   }
+  G[i + int64_t(L0) * j + P * kq] = g;
 }
 
 // legacy loops: osc_getgrnfree (core:476-493), osc_getgrnimageshift
@@ -149,11 +151,22 @@ void build_green_real(osc_ctx* c, const GreenSpec& s, double* g) {
     vmin = vmin + s.offset[1];
     wmin = wmin + s.offset[2] * s.gamma;
     DevBuf F(c->pool, sizeof(double) * N);
-    hipLaunchKernelGGL(k_prim_offset, dim3(nblk(N)), dim3(kThreads), 0, c->stream, F.as<double>(),
-                       s.L[0], s.L[1], s.L[2], dx, dy, dz, umin, vmin, wmin, factor, s.icomp);
+    const dim3 g3(unsigned((s.L[0] + 63) / 64), unsigned(s.L[1]), unsigned(s.L[2]));
+    // the solve path evaluates E components on half the grid along their own
+    // axis (mirror symmetry, see k_prim_offset); the returned-to-caller Green
+    // function (zero_raw == 0) keeps upstream's coordinate arithmetic
+    int mir = -1;
+    if (s.zero_raw && s.icomp >= 1 && s.offset[s.icomp - 1] == 0.0 && s.L[s.icomp - 1] % 2 == 0)
+      mir = s.icomp - 1;
+    dim3 gp = g3;
+    if (mir == 0) gp.x = unsigned((s.L[0] / 2 + 63) / 64);
+    if (mir == 1) gp.y = unsigned(s.L[1] / 2);
+    if (mir == 2) gp.z = unsigned(s.L[2] / 2);
+    hipLaunchKernelGGL(k_prim_offset, gp, dim3(64), 0, c->stream, F.as<double>(), s.L[0], s.L[1],
+                       s.L[2], dx, dy, dz, umin, vmin, wmin, factor, s.icomp, mir);
     OSC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_diff_offset, dim3(nblk(N)), dim3(kThreads), 0, c->stream,
-                       F.as<const double>(), g, s.L[0], s.L[1], s.L[2], s.out_shift_z, s.zero_raw);
+    hipLaunchKernelGGL(k_diff_offset, g3, dim3(64), 0, c->stream, F.as<const double>(), g, s.L[0],
+                       s.L[1], s.L[2], s.out_shift_z, s.zero_raw);
     OSC_HIP(hipGetLastError());
   } else if (s.kind == GK_PIPE) {
     hipLaunchKernelGGL(k_green_pipe, dim3(nblk(N)), dim3(kThreads), 0, c->stream, g, s);
@@ -164,7 +177,8 @@ void build_green_real(osc_ctx* c, const GreenSpec& s, double* g) {
   }
 }
 
-void spectrum_forward_full(osc_ctx* c, const double* g, cpx<double>* spec, const SpecDims& d) {
+template <typename T>
+void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const SpecDims& d) {
   const int L0 = d.L[0], L1 = d.L[1], L2 = d.L[2];
   ProfScope ps(c, ST_GREEN_FFT);
   R2CIO r{};
@@ -175,7 +189,7 @@ void spectrum_forward_full(osc_ctx* c, const double* g, cpx<double>* spec, const
   r.nrows = int64_t(L1) * L2;
   r.ny_rows = L1;
   r.n_in = L0;
-  launch_xr2c<double>(L0, g, spec, r, twiddles<double>(c, L0), c->stream);
+  launch_xr2c<T>(L0, g, spec, r, twiddles<T>(c, L0), c->stream);
   AxisIO ay{};
   ay.s_b = ay.d_b = 1;
   ay.s_p = ay.d_p = d.KXP;
@@ -184,7 +198,7 @@ void spectrum_forward_full(osc_ctx* c, const double* g, cpx<double>* spec, const
   ay.n_in = L1;
   ay.w0 = 0;
   ay.n_out = L1;
-  launch_axis<double>(L1, -1, spec, spec, ay, L2, 1, twiddles<double>(c, L1), c->stream);
+  launch_axis<T>(L1, -1, spec, spec, ay, L2, 1, twiddles<T>(c, L1), c->stream);
   AxisIO az{};
   az.s_b = az.d_b = 1;
   az.s_p = az.d_p = int64_t(d.KXP) * L1;
@@ -193,8 +207,10 @@ void spectrum_forward_full(osc_ctx* c, const double* g, cpx<double>* spec, const
   az.n_in = L2;
   az.w0 = 0;
   az.n_out = L2;
-  launch_axis<double>(L2, -1, spec, spec, az, L1, 1, twiddles<double>(c, L2), c->stream);
+  launch_axis<T>(L2, -1, spec, spec, az, L1, 1, twiddles<T>(c, L2), c->stream);
 }
+template void spectrum_forward_full<double>(osc_ctx*, const double*, cpx<double>*, const SpecDims&);
+template void spectrum_forward_full<float>(osc_ctx*, const double*, cpx<float>*, const SpecDims&);
 
 void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_t n) {
   ProfScope ps(c, ST_GREEN_FFT);
@@ -202,67 +218,57 @@ void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_
   OSC_HIP(hipGetLastError());
 }
 
+// Spectrum of the Green function of s in the context's precision: the real
+// Green function is always built in FP64 (IGF primitives and 8-corner
+// differences); an FP32 context transforms it in FP32.  Cached per
+// precision under the full specification.
 template <typename T>
 const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) {
   const SpecDims d = SpecDims::of(s.L);
   const int64_t N = int64_t(s.L[0]) * s.L[1] * s.L[2];
   std::string key(reinterpret_cast<const char*>(&s), sizeof(s));
   GreenCache& gc = c->gcache;
+  GreenCache::Entry* hit = nullptr;
   if (gc.enable) {
     auto it = gc.map.find(key);
     if (it != gc.map.end()) {
-      GreenCache::Entry& e = *it->second;
-      e.last = ++gc.tick;
-      ++gc.hits;
-      if constexpr (sizeof(T) == 4) {
-        if (!e.f32.p) {
-          e.f32 = DevBuf(c->pool, sizeof(cpx<float>) * d.size());
-          convert_spectrum(c, e.f64.as<cpx<double>>(), e.f32.as<cpx<float>>(), d.size());
-          e.bytes += sizeof(cpx<float>) * d.size();
-          gc.bytes += sizeof(cpx<float>) * d.size();
-        }
-        return e.f32.as<cpx<float>>();
-      } else {
-        return e.f64.as<cpx<double>>();
+      hit = it->second.get();
+      hit->last = ++gc.tick;
+      DevBuf& have = sizeof(T) == 4 ? hit->f32 : hit->f64;
+      if (have.p) {
+        ++gc.hits;
+        return have.as<cpx<T>>();
       }
     }
     ++gc.misses;
   }
   DevBuf g(c->pool, sizeof(double) * N);
   build_green_real(c, s, g.as<double>());
-  DevBuf sp64(c->pool, sizeof(cpx<double>) * d.size());
-  spectrum_forward_full(c, g.as<const double>(), sp64.as<cpx<double>>(), d);
+  DevBuf sp(c->pool, sizeof(cpx<T>) * d.size());
+  spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d);
   g.reset();
-  DevBuf sp32;
-  if constexpr (sizeof(T) == 4) {
-    sp32 = DevBuf(c->pool, sizeof(cpx<float>) * d.size());
-    convert_spectrum(c, sp64.as<cpx<double>>(), sp32.as<cpx<float>>(), d.size());
-  }
   if (!gc.enable) {
-    if constexpr (sizeof(T) == 4) {
-      keepalive = std::move(sp32);
-      return keepalive.as<cpx<float>>();
-    } else {
-      keepalive = std::move(sp64);
-      return keepalive.as<cpx<double>>();
-    }
+    keepalive = std::move(sp);
+    return keepalive.as<cpx<T>>();
   }
-  auto e = std::make_unique<GreenCache::Entry>();
-  e->bytes = sizeof(cpx<double>) * d.size() + (sp32.p ? sizeof(cpx<float>) * d.size() : 0);
-  e->f64 = std::move(sp64);
-  e->f32 = std::move(sp32);
-  e->last = ++gc.tick;
-  gc.bytes += e->bytes;
-  GreenCache::Entry* ep = e.get();
-  gc.map[key] = std::move(e);
+  GreenCache::Entry* ep = hit;
+  if (!ep) {
+    auto e = std::make_unique<GreenCache::Entry>();
+    ep = e.get();
+    gc.map[key] = std::move(e);
+  }
+  const size_t bytes = sizeof(cpx<T>) * d.size();
+  (sizeof(T) == 4 ? ep->f32 : ep->f64) = std::move(sp);
+  ep->bytes += bytes;
+  ep->last = ++gc.tick;
+  gc.bytes += bytes;
   // keep the entry being returned even if the budget is exceeded
   if (gc.bytes > gc.max_bytes) {
     ep->last = ~uint64_t(0);
     gc.evict_to(gc.max_bytes);
     ep->last = gc.tick;
   }
-  if constexpr (sizeof(T) == 4) return ep->f32.as<cpx<float>>();
-  else return ep->f64.as<cpx<double>>();
+  return (sizeof(T) == 4 ? ep->f32 : ep->f64).template as<cpx<T>>();
 }
 template const cpx<double>* green_spectrum<double>(osc_ctx*, const GreenSpec&, DevBuf&);
 template const cpx<float>* green_spectrum<float>(osc_ctx*, const GreenSpec&, DevBuf&);

@@ -90,6 +90,8 @@ enum Stage : int {
   ST_GATHER,
   ST_FD_B,
   ST_OTHER,
+  ST_GATHER_PACK,
+  ST_GATHER_SORT,
   ST_COUNT
 };
 
@@ -203,8 +205,9 @@ void build_green_real(osc_ctx* c, const GreenSpec& s, double* g);  // (L0,L1,L2)
 // half spectrum of a real-space Green array in precision T; cached by spec
 template <typename T>
 const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive);
-// forward R2C of a real (L0,L1,L2) array into a half spectrum (FP64)
-void spectrum_forward_full(osc_ctx* c, const double* g, cpx<double>* spec, const SpecDims& d);
+// forward R2C of a real (L0,L1,L2) FP64 array into a half spectrum of precision T
+template <typename T>
+void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const SpecDims& d);
 void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_t n);
 // download a half spectrum and expand it to the upstream full-array
 // convention F_s[g] (sign s[d] on axis d)
@@ -243,11 +246,12 @@ struct DepositArgs {
   int harness;          // 1: depose_rho_scalar rules (unit weights, scale at end)
   double scale_after;   // harness chrgpermacro
 };
-// particles in cell-sorted order, as left by a deposit (for a following gather)
+// particles in cell-sorted order, as left by a deposit (for a following
+// gather) or by sort_for_gather
 struct SortedParticles {
   int64_t n = 0;
-  DevBuf sv;          // original particle index of each sorted slot (u32)
-  DevBuf xs, ys, zs;  // coordinates in sorted order
+  DevBuf sv;    // original particle index of each sorted slot (u32)
+  DevBuf srec;  // 32-byte records (x, y, z, q) in sorted order (double4)
 };
 // rho (nx,ny,nz) on device; the dropped count is left in counter 0
 // (read_counter(c, 0)); keep (optional) receives the sorted particles
@@ -268,5 +272,9 @@ int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const dou
 // same, for the particles of the preceding deposit in their cell-sorted order
 void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
                         const SortedParticles& sp, double* E, double* B, int8_t* clamped);
+// stable sort of n points by the mesh cell their gather reads (the clamped
+// cell of interpolate_field), for a cache-friendly gather
+void sort_for_gather(osc_ctx* c, const InterpArgs& a, int64_t n, const double* x, const double* y,
+                     const double* z, SortedParticles* out);
 
 }  // namespace osc

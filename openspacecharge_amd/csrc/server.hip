@@ -1,0 +1,285 @@
+// Request loop for a stream of mixed space-charge solves on one GPU (no
+// upstream counterpart: upstream solves one bunch per call on the CPU).
+//
+// Workers: each owns an osc_ctx with its own HIP stream, buffer pool and
+// Green-spectrum cache, and a host thread that runs one request at a time
+// through the C-ABI (deposit -> solve -> gather), so requests on different
+// workers overlap on the device.  Requests are routed by a key of (solver,
+// mesh shape, mesh geometry when fixed, gamma, pipe size) so that repeated
+// geometries meet their cached Green spectra.  Admission: a request starts
+// only when the estimated device bytes of the requests running plus its own
+// fit the byte budget (free HBM at creation less a margin, or the caller's
+// limit); a request that alone exceeds the budget runs when nothing else does.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "osc_internal.h"
+
+#define OSC_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Job {
+  int64_t ticket;
+  osc_request r;
+  int64_t est;
+  Clock::time_point t_submit;
+};
+
+struct Done {
+  int status = OSC_OK;
+  double queued_ms = 0, run_ms = 0, total_ms = 0;
+  int64_t n_dropped = 0, n_clamped = 0;
+};
+
+uint64_t mix(uint64_t h, uint64_t v) {
+  h ^= v + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+  return h;
+}
+uint64_t dbits(double d) {
+  uint64_t u;
+  std::memcpy(&u, &d, sizeof(u));
+  return u;
+}
+
+// routing key: requests that can share Green spectra share a worker
+uint64_t route_key(const osc_request& r) {
+  uint64_t h = mix(0, uint64_t(r.solver));
+  for (int a = 0; a < 3; ++a) h = mix(h, uint64_t(r.nhi[a]));
+  h = mix(h, dbits(r.gamma));
+  if (!r.resize_mesh)
+    for (int a = 0; a < 3; ++a) h = mix(mix(h, dbits(r.min[a])), dbits(r.max[a]));
+  if (r.solver == OSC_SOLVE_PIPE) h = mix(mix(h, dbits(r.apipe)), dbits(r.bpipe));
+  return h;
+}
+
+// device bytes of one request: mesh arrays, doubled-grid half spectra (rho,
+// up to four Green terms, the inverse work buffer), the real Green build,
+// node records, and the particle sort
+int64_t estimate_bytes(const osc_request& r) {
+  const int64_t n = int64_t(r.nhi[0]) * r.nhi[1] * r.nhi[2];
+  const int64_t half = (int64_t(r.nhi[0]) + 1) * 4 * int64_t(r.nhi[1]) * r.nhi[2] * 16;
+  return 64 * n + 10 * half + 64 * n + 64 * n + 96 * r.n + (int64_t(64) << 20);
+}
+
+}  // namespace
+
+struct osc_server {
+  int device = 0;
+  int green_cache = 1;
+  int64_t budget = 0, reserved = 0;
+  std::vector<osc_ctx*> ctxs;
+  std::vector<std::thread> threads;
+  std::vector<std::deque<Job>> queues;
+  std::unordered_map<int64_t, Done> done;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  int64_t next_ticket = 1;
+  int64_t in_flight = 0;
+  bool stop = false;
+  std::string last_error;
+
+  void run_worker(int w);
+  int run_one(osc_ctx* c, const osc_request& r, Done& d);
+};
+
+int osc_server::run_one(osc_ctx* c, const osc_request& r, Done& d) {
+  using namespace osc;
+  int rc = osc_ctx_set_precision(c, r.precision);
+  if (rc) return rc;
+  osc_mesh m{};
+  for (int a = 0; a < 3; ++a) {
+    m.nlo[a] = 1;
+    m.nhi[a] = r.nhi[a];
+    m.npad[a] = 1;
+    m.min[a] = r.min[a];
+    m.max[a] = r.max[a];
+    m.delta[a] = r.nhi[a] > 1 ? (r.max[a] - r.min[a]) / double(r.nhi[a] - 1) : 1.0;
+  }
+  m.gamma = r.gamma;
+  const int64_t n = int64_t(r.nhi[0]) * r.nhi[1] * r.nhi[2];
+  const int want_b = r.B != nullptr;
+  DevBuf rho(c->pool, 8 * n), ef(c->pool, 24 * n), bf(c->pool, 24 * n);
+  const double tc = r.total_charge;
+  if (r.solver == OSC_SOLVE_FREE || r.solver == OSC_SOLVE_CATHODE) {
+    rc = osc_space_charge_step(c, r.n, r.x, r.y, r.z, nullptr, &tc, r.resize_mesh, nullptr,
+                               r.solver == OSC_SOLVE_CATHODE, want_b, &m, rho.as<double>(),
+                               ef.as<double>(), want_b ? bf.as<double>() : nullptr, r.E, r.B,
+                               &d.n_dropped, &d.n_clamped);
+    if (rc) return rc;
+  } else if (r.solver == OSC_SOLVE_PIPE) {
+    if (r.resize_mesh) {
+      c->last_error = "pipe requests need a fixed mesh (the pipe cross-section)";
+      return OSC_ERR_ARG;
+    }
+    rc = osc_deposit_particles(c, r.n, r.x, r.y, r.z, nullptr, &tc, 0, &m, rho.as<double>(),
+                               &d.n_dropped);
+    if (rc) return rc;
+    // the pipe Green functions stay in the context while the geometry repeats
+    const PipeState& ps = c->pipe;
+    bool same = ps.built && ps.a == r.apipe && ps.b == r.bpipe && ps.gamma == r.gamma;
+    for (int a = 0; a < 3 && same; ++a)
+      same = ps.nlo[a] == 1 && ps.nhi[a] == r.nhi[a] && ps.npad[a] == 1 &&
+             ps.delta[a] == m.delta[a] && ps.umin[a] == m.min[a];
+    if (!same) {
+      const int32_t one[3] = {1, 1, 1};
+      rc = osc_alloc_rectpipe_arrays(c, one, r.nhi, one);
+      if (!rc) rc = osc_getgrnpipe(c, r.gamma, r.apipe, r.bpipe, m.delta, m.min, one);
+      if (rc) return rc;
+    }
+    DevBuf phi(c->pool, 8 * n);
+    rc = osc_space_charge_rectpipe(c, &m, r.apipe, r.bpipe, 0, 1, rho.as<double>(),
+                                   phi.as<double>(), ef.as<double>(), bf.as<double>());
+    if (rc) return rc;
+    rc = osc_interpolate_field(c, &m, ef.as<double>(), want_b ? bf.as<double>() : nullptr, r.n,
+                               r.x, r.y, r.z, r.E, r.B, nullptr, &d.n_clamped);
+    if (rc) return rc;
+  } else {
+    c->last_error = "unknown solver";
+    return OSC_ERR_ARG;
+  }
+  return osc_ctx_synchronize(c);
+}
+
+void osc_server::run_worker(int w) {
+  osc_ctx* c = ctxs[w];
+  for (;;) {
+    Job job;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_work.wait(lk, [&] {
+        if (stop) return true;
+        if (queues[w].empty()) return false;
+        const int64_t est = queues[w].front().est;
+        return reserved == 0 || reserved + est <= budget;
+      });
+      if (stop && queues[w].empty()) return;
+      job = queues[w].front();
+      queues[w].pop_front();
+      reserved += job.est;
+    }
+    const auto t_start = Clock::now();
+    Done d;
+    d.status = run_one(c, job.r, d);
+    const auto t_end = Clock::now();
+    d.queued_ms = std::chrono::duration<double, std::milli>(t_start - job.t_submit).count();
+    d.run_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
+    d.total_ms = std::chrono::duration<double, std::milli>(t_end - job.t_submit).count();
+    // pooled buffers beyond a share of the budget go back to the device
+    if (int64_t(c->pool.bytes_cached()) > budget / int64_t(2 * ctxs.size())) c->pool.release_free();
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (d.status != OSC_OK) last_error = c->last_error;
+      reserved -= job.est;
+      --in_flight;
+      done[job.ticket] = d;
+    }
+    cv_work.notify_all();
+    cv_done.notify_all();
+  }
+}
+
+OSC_API int osc_server_create(int device, int nworkers, int64_t max_bytes, int green_cache,
+                              osc_server** out) {
+  if (!out || nworkers < 1 || nworkers > 16) return OSC_ERR_ARG;
+  *out = nullptr;
+  auto* s = new osc_server();
+  s->device = device;
+  s->green_cache = green_cache;
+  for (int w = 0; w < nworkers; ++w) {
+    osc_ctx* c = nullptr;
+    const int rc = osc_ctx_create(device, &c);
+    if (rc) {
+      for (auto* cc : s->ctxs) osc_ctx_destroy(cc);
+      delete s;
+      return rc;
+    }
+    osc_ctx_set_green_cache(c, green_cache, 0);
+    s->ctxs.push_back(c);
+  }
+  size_t fr = 0, tot = 0;
+  {
+    osc::DeviceGuard g(device);
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      (void)hipGetLastError();
+      fr = size_t(16) << 30;
+    }
+  }
+  const int64_t avail = int64_t(fr) - int64_t(fr / 8);  // keep an eighth free
+  s->budget = max_bytes > 0 ? std::min<int64_t>(max_bytes, avail) : avail;
+  s->queues.resize(nworkers);
+  for (int w = 0; w < nworkers; ++w) s->threads.emplace_back([s, w] { s->run_worker(w); });
+  *out = s;
+  return OSC_OK;
+}
+
+OSC_API int osc_server_submit(osc_server* s, const osc_request* r, int64_t* ticket) {
+  if (!s || !r || !ticket) return OSC_ERR_ARG;
+  if (r->n < 1 || !r->x || !r->y || !r->z || !r->E || r->nhi[0] < 2 || r->nhi[1] < 2 ||
+      r->nhi[2] < 2)
+    return OSC_ERR_ARG;
+  Job j;
+  j.r = *r;
+  j.est = estimate_bytes(*r);
+  j.t_submit = Clock::now();
+  const size_t w = size_t(route_key(*r) % s->ctxs.size());
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    j.ticket = s->next_ticket++;
+    *ticket = j.ticket;
+    ++s->in_flight;
+    s->queues[w].push_back(j);
+  }
+  s->cv_work.notify_all();
+  return OSC_OK;
+}
+
+OSC_API int osc_server_wait(osc_server* s, int64_t ticket, double* times_ms, int64_t* counts) {
+  if (!s) return OSC_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (ticket < 1 || ticket >= s->next_ticket) return OSC_ERR_ARG;
+  s->cv_done.wait(lk, [&] { return s->done.count(ticket) > 0; });
+  const Done d = s->done[ticket];
+  s->done.erase(ticket);
+  if (times_ms) {
+    times_ms[0] = d.queued_ms;
+    times_ms[1] = d.run_ms;
+    times_ms[2] = d.total_ms;
+  }
+  if (counts) {
+    counts[0] = d.n_dropped;
+    counts[1] = d.n_clamped;
+  }
+  return d.status;
+}
+
+OSC_API int osc_server_drain(osc_server* s) {
+  if (!s) return OSC_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [&] { return s->in_flight == 0; });
+  return OSC_OK;
+}
+
+OSC_API const char* osc_server_last_error(const osc_server* s) {
+  return s ? s->last_error.c_str() : "null server";
+}
+
+OSC_API int osc_server_destroy(osc_server* s) {
+  if (!s) return OSC_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  for (auto& t : s->threads) t.join();
+  for (auto* c : s->ctxs) osc_ctx_destroy(c);
+  delete s;
+  return OSC_OK;
+}

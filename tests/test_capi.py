@@ -106,6 +106,23 @@ def test_mesh_struct_layout(tmp_path):
     assert mine == vals
 
 
+def test_request_struct_layout(tmp_path):
+    """ctypes OscRequest matches osc_request."""
+    fields = [f for f, _ in _capi.OscRequest._fields_]
+    src = tmp_path / "req.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "opensc_hip.h"\n'
+                   "int main(void){printf(\"%zu\\n\", sizeof(osc_request));" +
+                   "".join(f'printf("%zu\\n", offsetof(osc_request,{f}));' for f in fields) +
+                   "return 0;}\n")
+    exe = tmp_path / "req"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)],
+                   check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                           check=True).stdout.split()]
+    R = _capi.OscRequest
+    assert [C.sizeof(R)] + [getattr(R, f).offset for f in fields] == vals
+
+
 def prime_factors(n):
     out, p = set(), 2
     while p * p <= n:
