@@ -84,7 +84,11 @@ def stage_bytes(stage, s, n3, L, P, nout, nspec):
     if stage == "fft.inv_x":
         return nout * c * KX * ny * nz + 8 * n * 5
     if stage == "gather":
+        # compulsory part of the field gather (the unpermute pass is separate)
         return 24 * P + 48 * P + 48 * n
+    if stage == "gather.out":
+        # one 64-byte record per particle in, E and B (48 B) out, 4 B index
+        return 64 * P + 48 * P + 4 * P
     return None
 
 
@@ -386,7 +390,7 @@ def cpu_child(args):
 
 
 # stage of the per-stage HIP-event timing -> the one kernel it times
-STAGE_KERNEL = {"gather": "k_interp_rec<true>", "fft.inv_zmul": "k_zinv<double",
+STAGE_KERNEL = {"gather": "k_interp_cells", "gather.out": "k_unpermute", "fft.inv_zmul": "k_zinv<double",
                 "fft.inv_x": "k_xc2r<double", "fft.fwd_x": "k_xr2c<double"}
 
 

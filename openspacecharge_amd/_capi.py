@@ -159,6 +159,15 @@ def load(path: str = LIB_PATH) -> C.CDLL:
             raise LibraryNotBuilt(
                 f"{path} is not built; run `python -m openspacecharge_amd.build` (or "
                 "__graft_entry__.build()). There is no CPU fallback.")
+        # torch ships its own HIP and HSA runtime libraries under the same
+        # sonames as ROCm's; the first loaded serves the whole process, and
+        # torch cannot initialise the GPU on a runtime it was not built with.
+        # Loading torch first keeps one runtime for both (and lets torch
+        # tensors be passed as device arrays).
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
         lib = C.CDLL(path)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(lib, name)
@@ -217,7 +226,7 @@ class Context:
     def synchronize(self):
         self.check(self.lib.osc_ctx_synchronize(self.h))
 
-    NSTAGE = 18
+    NSTAGE = 19
 
     def set_profiling(self, on: bool):
         self.check(self.lib.osc_ctx_set_profiling(self.h, int(bool(on))))

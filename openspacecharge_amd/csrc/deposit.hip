@@ -205,6 +205,7 @@ __global__ void __launch_bounds__(RS_T)
 struct SortPayload {
   const double4* rec;
   double4* srec;
+  uint32_t* inv;  // inv[original index] = sorted slot (may be null)
 };
 
 __global__ void __launch_bounds__(RS_T)
@@ -242,6 +243,7 @@ __global__ void __launch_bounds__(RS_T)
       kout[pos] = k;
       vout[pos] = v;
       if (pl.srec) pl.srec[pos] = pl.rec[v];
+      if (pl.inv) pl.inv[v] = pos;
     }
     __syncthreads();
     for (int dd = threadIdx.x; dd <= int(mask); dd += RS_T) {
@@ -650,7 +652,8 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   int nbits = 0;
   while ((uint64_t(1) << nbits) <= uint64_t(ncell)) ++nbits;
   bool in_b = false;
-  SortPayload pl{rec.as<const double4>(), srec.as<double4>()};
+  DevBuf inv(c->pool, keep ? sizeof(uint32_t) * (n + 1) : 0);
+  SortPayload pl{rec.as<const double4>(), srec.as<double4>(), keep ? inv.as<uint32_t>() : nullptr};
   {
     ProfScope ps(c, ST_DEP_SORT);
     radix_sort(c, ka, va, kb, vb, n, nbits, pl, in_b);
@@ -688,6 +691,7 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
     keep->n = n;
     keep->sv = std::move(in_b ? vb : va);
     keep->srec = std::move(srec);
+    keep->inv = std::move(inv);
   }
 }
 
@@ -720,9 +724,11 @@ void sort_for_gather(osc_ctx* c, const InterpArgs& a, int64_t n, const double* x
   int nbits = 0;
   while ((uint64_t(1) << nbits) <= uint64_t(ncell)) ++nbits;
   bool in_b = false;
-  SortPayload pl{rec.as<const double4>(), srec.as<double4>()};
+  DevBuf inv(c->pool, sizeof(uint32_t) * (n + 1));
+  SortPayload pl{rec.as<const double4>(), srec.as<double4>(), inv.as<uint32_t>()};
   radix_sort(c, ka, va, kb, vb, n, nbits, pl, in_b);
   out->n = n;
+  out->inv = std::move(inv);
   out->sv = std::move(in_b ? vb : va);
   out->srec = std::move(srec);
 }

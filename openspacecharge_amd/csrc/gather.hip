@@ -166,6 +166,73 @@ __global__ void k_interp_rec(const double* __restrict__ R, int wantE, int wantB,
   __syncthreads();
   if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
 }
+// cell-ordered gather: slot s (particle sv[s], record srec[s]) leaves
+// (E, B, 0, 0) in the 64-byte record out8[s], written contiguously; the
+// values are those of k_interp_rec (same terms, same order)
+__global__ void k_interp_cells(const double* __restrict__ R, int wantE, int wantB, int64_t n,
+                               const double4* __restrict__ srec, const uint32_t* __restrict__ sv,
+                               IP p, double* __restrict__ out8, int8_t* __restrict__ clamped,
+                               uint32_t* __restrict__ nclamp) {
+  __shared__ uint32_t cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  uint32_t mine = 0;
+  const int64_t sy = p.nn[0], sz = int64_t(p.nn[0]) * p.nn[1];
+  for (int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; s < n;
+       s += int64_t(gridDim.x) * blockDim.x) {
+    const double4 r = srec[s];
+    const double xi = r.x, yi = r.y, zi = r.z;
+    bool fl = false;
+    const int ip = clamp_idx(xi, p.mn[0], p.hinv[0], p.nhi[0], fl);
+    const int jp = clamp_idx(yi, p.mn[1], p.hinv[1], p.nhi[1], fl);
+    const int kp = clamp_idx(zi, p.mn[2], p.hinv[2], p.nhi[2], fl);
+    const double ab = ((p.mn[0] - xi) + ip * p.delta[0]) * p.hinv[0];
+    const double de = ((p.mn[1] - yi) + jp * p.delta[1]) * p.hinv[1];
+    const double gh = ((p.mn[2] - zi) + kp * p.delta[2]) * p.hinv[2];
+    int i0 = ip - p.nlo[0], j0 = jp - p.nlo[1], k0 = kp - p.nlo[2];
+    i0 = i0 < 0 ? 0 : (i0 > p.nn[0] - 2 ? p.nn[0] - 2 : i0);
+    j0 = j0 < 0 ? 0 : (j0 > p.nn[1] - 2 ? p.nn[1] - 2 : j0);
+    k0 = k0 < 0 ? 0 : (k0 > p.nn[2] - 2 ? p.nn[2] - 2 : k0);
+    const int64_t b = i0 + j0 * sy + k0 * sz;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (wantE)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = wsum8(R, sy, sz, b, c, ab, de, gh);
+    if (wantB)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[3 + c] = wsum8(R, sy, sz, b, 3 + c, ab, de, gh);
+    double4* o = reinterpret_cast<double4*>(out8 + 8 * s);
+    o[0] = make_double4(v[0], v[1], v[2], v[3]);
+    o[1] = make_double4(v[4], v[5], 0.0, 0.0);
+    if (clamped) clamped[sv[s]] = fl ? 1 : 0;
+    mine += fl ? 1u : 0u;
+  }
+  if (mine) atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
+}
+
+// original order: particle i reads its slot inv[i] (one 64-byte record) and
+// writes E(3,n) and B(3,n) contiguously -- a random read costs one line,
+// while a random 24-byte write costs two partial sectors
+__global__ void k_unpermute(const double* __restrict__ out8, const uint32_t* __restrict__ inv,
+                            int64_t n, double* __restrict__ E, double* __restrict__ B) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double4* r = reinterpret_cast<const double4*>(out8 + 8 * int64_t(inv[i]));
+    const double4 a = r[0], b = r[1];
+    if (E) {
+      E[3 * i] = a.x;
+      E[3 * i + 1] = a.y;
+      E[3 * i + 2] = a.z;
+    }
+    if (B) {
+      B[3 * i] = a.w;
+      B[3 * i + 1] = b.x;
+      B[3 * i + 2] = b.y;
+    }
+  }
+}
 }  // namespace
 
 void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
@@ -191,15 +258,22 @@ void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const
                        rec.as<double>());
     OSC_HIP(hipGetLastError());
   }
-  ProfScope ps(c, ST_GATHER);
   const int64_t n = sp.n;
   if (n > 0) {
+    DevBuf out8(c->pool, sizeof(double) * 8 * n);
     nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
-    hipLaunchKernelGGL(k_interp_rec<true>, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
-                       rec.as<const double>(), int(E && ef), int(B && bf), n,
-                       static_cast<const double*>(nullptr), static_cast<const double*>(nullptr),
-                       static_cast<const double*>(nullptr), sp.srec.as<const double4>(),
-                       sp.sv.as<const uint32_t>(), p, E, B, clamped, cnt);
+    {
+      ProfScope ps(c, ST_GATHER);
+      hipLaunchKernelGGL(k_interp_cells, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
+                         rec.as<const double>(), int(E && ef), int(B && bf), n,
+                         sp.srec.as<const double4>(), sp.sv.as<const uint32_t>(), p,
+                         out8.as<double>(), clamped, cnt);
+      OSC_HIP(hipGetLastError());
+    }
+    ProfScope po(c, ST_GATHER_OUT);
+    hipLaunchKernelGGL(k_unpermute, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
+                       out8.as<const double>(), sp.inv.as<const uint32_t>(), n, (E && ef) ? E : nullptr,
+                       (B && bf) ? B : nullptr);
     OSC_HIP(hipGetLastError());
   }
 }

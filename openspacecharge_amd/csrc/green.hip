@@ -97,34 +97,82 @@ __global__ void k_green_legacy(double* __restrict__ G, GreenSpec s) {
 }
 
 // osc_getgrnpipe loops (core:652-713), term t = 0..3
-__global__ void k_green_pipe(double* __restrict__ G, GreenSpec s) {
-  const int L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
-  const int64_t N = int64_t(L0) * L1 * L2;
-  const double hx = s.delta[0], hy = s.delta[1], hz = s.delta[2];
+// rfun (core:723-755) factorised over the grid: the cosines depend only on
+// the x (resp. y) index and the mode, the z profile only on the z index and
+// the mode pair.  k_pipe_tables evaluates them once with the expressions of
+// rfun; k_green_pipe then forms every term and the sum in rfun's order, so
+// each value equals rfun(u, v, w, ...) evaluated point by point.
+constexpr double kPi = 3.141592653589793115997963468544185161590576171875;  // acos(-1d0)
+
+__device__ __forceinline__ double pipe_u(const GreenSpec& s, int ii, int si) {
+  if (si == 1) return double(s.lo[0] + (ii + s.shift[0]) % s.L[0]) * s.delta[0];
+  return 2.0 * s.umin[0] + ii * s.delta[0];
+}
+__device__ __forceinline__ double pipe_v(const GreenSpec& s, int jj, int sj) {
+  if (sj == 1) return double(s.lo[1] + (jj + s.shift[1]) % s.L[1]) * s.delta[1];
+  return 2.0 * s.umin[1] + jj * s.delta[1];
+}
+
+// cu[ii*5 + m-1] = cos(m u piainv), cv[jj*5 + n-1] = cos(n v pibinv),
+// zf[kk*25 + (m-1)*5 + n-1] = zfun(w), kap[(m-1)*5 + n-1] = kapmn
+__global__ void k_pipe_tables(GreenSpec s, int si, int sj, double* __restrict__ cu,
+                              double* __restrict__ cv, double* __restrict__ zf,
+                              double* __restrict__ kap) {
+  const double ainv = 1.0 / s.a, binv = 1.0 / s.b;
+  const double piainv = kPi * ainv, pibinv = kPi * binv;
+  const double hz = s.delta[2], gam = s.gamma;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < s.L[0]) {
+    const double u = pipe_u(s, t, si);
+    for (int m = 1; m <= 5; ++m) cu[t * 5 + m - 1] = cos(m * u * piainv);
+  }
+  if (t < s.L[1]) {
+    const double v = pipe_v(s, t, sj);
+    for (int n = 1; n <= 5; ++n) cv[t * 5 + n - 1] = cos(n * v * pibinv);
+  }
+  if (t < s.L[2]) {
+    const int kp = s.lo[2] + (t + s.shift[2]) % s.L[2];
+    const double w = kp * hz;
+    for (int m = 1; m <= 5; ++m)
+      for (int n = 1; n <= 5; ++n) {
+        const double km = m * kPi * ainv, kn = n * kPi * binv;
+        const double kapmn = sqrt(km * km + kn * kn);
+        double zfun = (exp(-(kapmn * fabs(gam * w - hz))) - 2.0 * exp(-(kapmn * fabs(gam * w))) +
+                       exp(-(kapmn * fabs(gam * w + hz)))) /
+                      (hz * hz * (kapmn * kapmn));
+        if (w == 0) zfun = zfun + 2.0 / (hz * kapmn);
+        zf[t * 25 + (m - 1) * 5 + n - 1] = zfun;
+        if (t == 0) kap[(m - 1) * 5 + n - 1] = kapmn;
+      }
+  }
+}
+
+// osc_getgrnpipe loops (core:652-713) for one of the four terms
+// (s.term: bit 1 -> image in x, bit 0 -> image in y); grid (ceil(L0/64), L1, L2)
+__global__ void __launch_bounds__(64)
+    k_green_pipe(double* __restrict__ G, GreenSpec s, const double* __restrict__ cu,
+                 const double* __restrict__ cv, const double* __restrict__ zf,
+                 const double* __restrict__ kap) {
+  const int ii = blockIdx.x * 64 + threadIdx.x, jj = blockIdx.y, kk = blockIdx.z;
+  if (ii >= s.L[0]) return;
   const int si = (s.term & 2) ? -1 : 1;
   const int sj = (s.term & 1) ? -1 : 1;
-  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < N;
-       idx += int64_t(gridDim.x) * blockDim.x) {
-    const int ii = int(idx % L0);
-    const int jj = int((idx / L0) % L1);
-    const int kk = int(idx / (int64_t(L0) * L1));
-    const int kp = s.lo[2] + (kk + s.shift[2]) % L2;
-    const double w = kp * hz;
-    double v, u;
-    if (sj == 1) {
-      const int jp = s.lo[1] + (jj + s.shift[1]) % L1;
-      v = jp * hy;
-    } else {
-      v = 2.0 * s.umin[1] + jj * hy;
+  const double ainv = 1.0 / s.a, binv = 1.0 / s.b;
+  double res = 0.0;
+  int im = 1;
+  for (int m = 1; m <= 5; ++m) {
+    im *= si;  // i**m
+    const double cm = cu[ii * 5 + m - 1];
+    int jn = 1;
+    for (int n = 1; n <= 5; ++n) {
+      jn *= sj;  // j**n
+      const double term = double(im * jn) * cm * cv[jj * 5 + n - 1] * zf[kk * 25 + (m - 1) * 5 + n - 1] /
+                          kap[(m - 1) * 5 + n - 1];
+      res = res + term;
     }
-    if (si == 1) {
-      const int ip = s.lo[0] + (ii + s.shift[0]) % L0;
-      u = ip * hx;
-    } else {
-      u = 2.0 * s.umin[0] + ii * hx;
-    }
-    G[idx] = rfun(u, v, w, s.gamma, s.a, s.b, hz, si, sj);
   }
+  res = res * 2.0 * kPi * ainv * binv;
+  G[ii + int64_t(s.L[0]) * (jj + int64_t(s.L[1]) * kk)] = res;
 }
 
 __global__ void k_convert(const cpx<double>* __restrict__ in, cpx<float>* __restrict__ out,
@@ -169,7 +217,18 @@ void build_green_real(osc_ctx* c, const GreenSpec& s, double* g) {
                        s.L[1], s.L[2], s.out_shift_z, s.zero_raw);
     OSC_HIP(hipGetLastError());
   } else if (s.kind == GK_PIPE) {
-    hipLaunchKernelGGL(k_green_pipe, dim3(nblk(N)), dim3(kThreads), 0, c->stream, g, s);
+    const int si = (s.term & 2) ? -1 : 1, sj = (s.term & 1) ? -1 : 1;
+    const int lmax = std::max(s.L[0], std::max(s.L[1], s.L[2]));
+    DevBuf tab(c->pool, sizeof(double) * (5 * (s.L[0] + s.L[1]) + 25 * s.L[2] + 25));
+    double* cu = tab.as<double>();
+    double* cv = cu + 5 * s.L[0];
+    double* zf = cv + 5 * s.L[1];
+    double* kap = zf + 25 * s.L[2];
+    hipLaunchKernelGGL(k_pipe_tables, dim3((lmax + 63) / 64), dim3(64), 0, c->stream, s, si, sj, cu,
+                       cv, zf, kap);
+    OSC_HIP(hipGetLastError());
+    const dim3 g3(unsigned((s.L[0] + 63) / 64), unsigned(s.L[1]), unsigned(s.L[2]));
+    hipLaunchKernelGGL(k_green_pipe, g3, dim3(64), 0, c->stream, g, s, cu, cv, zf, kap);
     OSC_HIP(hipGetLastError());
   } else {
     hipLaunchKernelGGL(k_green_legacy, dim3(nblk(N)), dim3(kThreads), 0, c->stream, g, s);

@@ -92,6 +92,7 @@ enum Stage : int {
   ST_OTHER,
   ST_GATHER_PACK,
   ST_GATHER_SORT,
+  ST_GATHER_OUT,
   ST_COUNT
 };
 
@@ -252,6 +253,7 @@ struct SortedParticles {
   int64_t n = 0;
   DevBuf sv;    // original particle index of each sorted slot (u32)
   DevBuf srec;  // 32-byte records (x, y, z, q) in sorted order (double4)
+  DevBuf inv;   // sorted slot of each original particle index (u32)
 };
 // rho (nx,ny,nz) on device; the dropped count is left in counter 0
 // (read_counter(c, 0)); keep (optional) receives the sorted particles
