@@ -19,12 +19,15 @@ static size_t bucket(size_t bytes) {
 
 void* DevPool::get(size_t bytes) {
   size_t bb = bucket(bytes);
-  auto it = free_.find(bb);
-  if (it != free_.end()) {
+  // best fit among cached buffers of at most twice the request, so that
+  // requests of varying sizes (a stream of bunches) reuse memory
+  auto it = free_.lower_bound(bb);
+  if (it != free_.end() && it->first <= 2 * bb) {
     void* p = it->second;
+    const size_t have = it->first;
     free_.erase(it);
-    cached_ -= bb;
-    live_[p] = bb;
+    cached_ -= have;
+    live_[p] = have;
     return p;
   }
   void* p = nullptr;

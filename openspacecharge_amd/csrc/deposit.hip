@@ -201,24 +201,51 @@ __global__ void __launch_bounds__(RS_T)
   for (int d = threadIdx.x; d <= int(mask); d += RS_T) hist[int64_t(d) * nb + blockIdx.x] = h[d];
 }
 
-// payload gathered by the last pass: the particle records in sorted order
+// payload gathered after the last pass: the particle records in sorted order
 struct SortPayload {
   const double4* rec;
   double4* srec;
   uint32_t* inv;  // inv[original index] = sorted slot (may be null)
 };
 
+// One pass: each block ranks its tile of RS_TILE pairs by digit (stable: lane
+// order within a wave, wave order, then iteration order), stages the
+// tile in LDS in that order, and writes it out so that consecutive threads
+// store consecutive slots of one digit's run.  offs[d * nb + b] is the
+// exclusive scan (digit-major) of the per-block digit counts, so the count of
+// (d, b) is the difference to the next entry (n for the last one).
 __global__ void __launch_bounds__(RS_T)
     k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                  uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n, int shift,
-                 int bits, const uint32_t* __restrict__ offs, int nb, SortPayload pl) {
-  __shared__ uint32_t run[256];
+                 int bits, const uint32_t* __restrict__ offs, int nb) {
+  __shared__ uint32_t lk[RS_TILE], lv[RS_TILE];
+  __shared__ uint32_t run[256], gofs[256], scan[256];
   __shared__ uint32_t wc[RS_W][256];
   const uint32_t mask = (1u << bits) - 1u;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
-  for (int d = threadIdx.x; d < 256; d += RS_T)
-    run[d] = d <= int(mask) ? offs[int64_t(d) * nb + blockIdx.x] : 0u;
+  const int64_t nflat = int64_t(mask + 1) * nb;
+  {
+    // this block's count of each digit, and their exclusive scan over digits
+    const int d = threadIdx.x;  // RS_T == 256 digits at most
+    uint32_t cnt = 0, g = 0;
+    if (d <= int(mask)) {
+      const int64_t f = int64_t(d) * nb + blockIdx.x;
+      g = offs[f];
+      cnt = (f + 1 < nflat ? offs[f + 1] : uint32_t(n)) - g;
+    }
+    scan[d] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const uint32_t a = d >= off ? scan[d - off] : 0u;
+      __syncthreads();
+      scan[d] += a;
+      __syncthreads();
+    }
+    const uint32_t start = scan[d] - cnt;  // exclusive
+    run[d] = start;
+    gofs[d] = g - start;  // global slot of local slot j of digit d: gofs[d] + j
+  }
   const int64_t base = int64_t(blockIdx.x) * RS_TILE;
   for (int it = 0; it < RS_I; ++it) {
     for (int j = threadIdx.x; j < RS_W * 256; j += RS_T) (&wc[0][0])[j] = 0u;
@@ -240,18 +267,35 @@ __global__ void __launch_bounds__(RS_T)
     if (valid) {
       uint32_t pos = run[d] + rank;
       for (int w = 0; w < wave; ++w) pos += wc[w][d];
-      kout[pos] = k;
-      vout[pos] = v;
-      if (pl.srec) pl.srec[pos] = pl.rec[v];
-      if (pl.inv) pl.inv[v] = pos;
+      lk[pos] = k;
+      lv[pos] = v;
     }
     __syncthreads();
     for (int dd = threadIdx.x; dd <= int(mask); dd += RS_T) {
-      uint32_t s = 0;
-      for (int w = 0; w < RS_W; ++w) s += wc[w][dd];
-      run[dd] += s;
+      uint32_t t = 0;
+      for (int w = 0; w < RS_W; ++w) t += wc[w][dd];
+      run[dd] += t;
     }
     __syncthreads();
+  }
+  const int64_t rem = n - base;
+  const int cnt = rem < RS_TILE ? int(rem) : RS_TILE;
+  for (int j = threadIdx.x; j < cnt; j += RS_T) {
+    const uint32_t k = lk[j];
+    const uint32_t g = gofs[(k >> shift) & mask] + uint32_t(j);
+    kout[g] = k;
+    vout[g] = lv[j];
+  }
+}
+
+// records in sorted order (slot s holds particle sv[s]) and, optionally, the
+// inverse permutation
+__global__ void k_sort_payload(const uint32_t* __restrict__ sv, int64_t n, SortPayload pl) {
+  for (int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; s < n;
+       s += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t v = sv[s];
+    pl.srec[s] = pl.rec[v];
+    if (pl.inv) pl.inv[v] = uint32_t(s);
   }
 }
 
@@ -570,10 +614,10 @@ void fill_u32(osc_ctx* c, uint32_t* p, int64_t n, uint32_t v) {
   OSC_HIP(hipGetLastError());
 }
 
-// Sorts (key, value) pairs; values start as the input index.  The last pass
-// also writes the payload (coordinates, charges) in sorted order.  When the
-// key range needs no pass the pairs are already "sorted" and the payload is
-// copied through a one-pass scatter with a zero-bit digit.
+// Sorts (key, value) pairs; values start as the input index.  The payload
+// (records, inverse permutation) is written from the sorted values at the
+// end.  When the key range needs no pass the pairs are already "sorted" and go
+// through a one-pass scatter with a zero-bit digit.
 static void radix_sort(osc_ctx* c, DevBuf& ka, DevBuf& va, DevBuf& kb, DevBuf& vb, int64_t n,
                        int nbits, const SortPayload& pl, bool& result_in_b) {
   int npass = nbits <= 0 ? 0 : (nbits + 7) / 8;
@@ -593,15 +637,18 @@ static void radix_sort(osc_ctx* c, DevBuf& ka, DevBuf& va, DevBuf& kb, DevBuf& v
                        hist.as<uint32_t>(), nb);
     OSC_HIP(hipGetLastError());
     exclusive_scan(c, hist.as<const uint32_t>(), offs.as<uint32_t>(), int64_t(mask + 1) * nb);
-    SortPayload none{};
     hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_T), 0, c->stream, kin,
                        p == 0 ? nullptr : vin, kout, vout, n, shift, bits,
-                       offs.as<const uint32_t>(), nb, p == npass - 1 ? pl : none);
+                       offs.as<const uint32_t>(), nb);
     OSC_HIP(hipGetLastError());
     std::swap(kin, kout);
     std::swap(vin, vout);
     result_in_b = !result_in_b;
     shift += bits;
+  }
+  if (pl.srec) {
+    hipLaunchKernelGGL(k_sort_payload, dim3(nblk(n)), dim3(kThreads), 0, c->stream, vin, n, pl);
+    OSC_HIP(hipGetLastError());
   }
 }
 

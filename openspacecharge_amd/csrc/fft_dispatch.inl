@@ -25,10 +25,14 @@ void axis_t(int dir, const cpx<T>* in, cpx<T>* out, const AxisIO& a, int g1, int
     hipLaunchKernelGGL((k_axis<T, L, 1>), g, dim3(NT), 0, s, in, out, a, tw);
 }
 template <int L>
-void xr2c_t(const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw, hipStream_t s) {
+void xr2c_t(const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw, hipStream_t s,
+            const CornerDiff* cd) {
   constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
   dim3 g = grid3(cdiv(cdiv(a.nrows, 2), BX), 1, 1);
-  hipLaunchKernelGGL((k_xr2c<T, L>), g, dim3(NT), 0, s, in, out, a, tw);
+  if (cd)
+    hipLaunchKernelGGL((k_xr2c<T, L, true>), g, dim3(NT), 0, s, in, out, a, tw, *cd);
+  else
+    hipLaunchKernelGGL((k_xr2c<T, L, false>), g, dim3(NT), 0, s, in, out, a, tw, CornerDiff{});
 }
 template <int L>
 void xc2r_t(const cpx<T>* in, const C2RIO<T>& a, int nout, const cpx<T>* tw, hipStream_t s) {
@@ -73,9 +77,9 @@ template <>
 void launch_xr2c<T>(int L, const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw,
                     hipStream_t s) {
   switch (L) {
-#define OSC_CASE(LL)                    \
-  case LL:                              \
-    xr2c_t<LL>(in, out, a, tw, s);      \
+#define OSC_CASE(LL)                         \
+  case LL:                                   \
+    xr2c_t<LL>(in, out, a, tw, s, nullptr);  \
     break;
     OSC_FOR_EACH_L(OSC_CASE)
 #undef OSC_CASE
@@ -85,6 +89,22 @@ void launch_xr2c<T>(int L, const double* in, cpx<T>* out, const R2CIO& a, const 
       hipLaunchKernelGGL((k_xr2c_gen<T>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in, out,
                          a, tw, pl);
     }
+  }
+  OSC_HIP(hipGetLastError());
+}
+
+template <>
+void launch_xr2c_diff<T>(int L, const double* F, cpx<T>* out, const R2CIO& a,
+                         const CornerDiff& cd, const cpx<T>* tw, hipStream_t s) {
+  switch (L) {
+#define OSC_CASE(LL)                     \
+  case LL:                               \
+    xr2c_t<LL>(F, out, a, tw, s, &cd);   \
+    break;
+    OSC_FOR_EACH_L(OSC_CASE)
+#undef OSC_CASE
+    default:
+      OSC_REQUIRE(false, OSC_ERR_FFT_LENGTH, "corner-difference x pass needs a planned length");
   }
   OSC_HIP(hipGetLastError());
 }

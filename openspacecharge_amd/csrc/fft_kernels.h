@@ -158,23 +158,69 @@ __device__ __forceinline__ void r2c_unpack(cpx<T> zk, cpx<T> zm, cpx<T>& A, cpx<
   B = mk(T(0.5) * d.y, T(-0.5) * d.x);
 }
 
-template <typename T, int L>
+// The real rows of the x pass taken as the 8-corner difference of a primitive
+// F (open_spacecharge_mod.f90:697-705): row (j, kq) of the result is built
+// from rows (j, k), (j+1, k), (j, k+1), (j+1, k+1) of F with
+// k = (kq - sh) mod L2; elements on the last plane of any axis keep the raw
+// primitive, or 0 with zero_raw.  F has the strides of R2CIO's input.
+struct CornerDiff {
+  int L2, sh, zero_raw, pad_;
+};
+
+// row r of the corner-difference source: its F row and whether it is interior
+__device__ __forceinline__ const double* diff_row(const double* F, const R2CIO& a,
+                                                  const CornerDiff& cd, int64_t r, bool ok,
+                                                  bool& inner) {
+  inner = false;
+  if (!ok) return F;
+  const int j = int(r % a.ny_rows);
+  int k = int(r / a.ny_rows) - cd.sh;
+  k %= cd.L2;
+  if (k < 0) k += cd.L2;
+  inner = j < a.ny_rows - 1 && k < cd.L2 - 1;
+  return F + j * a.in_sy + k * a.in_sz;
+}
+// element p of a corner-difference row, with the terms in the order of the
+// upstream loop (a left-to-right chain of adds and subtracts)
+__device__ __forceinline__ double diff_elem(const double* q, const R2CIO& a, const CornerDiff& cd,
+                                           bool inner, int p) {
+  if (inner && p < a.n_in - 1) {
+    const double* q10 = q + a.in_sy;
+    const double* q01 = q + a.in_sz;
+    const double* q11 = q01 + a.in_sy;
+    return q11[p + 1] - q11[p] - q01[p + 1] - q10[p + 1] - q[p] + q01[p] + q10[p] + q[p + 1];
+  }
+  return cd.zero_raw ? 0.0 : q[p];
+}
+
+template <typename T, int L, bool DIFF>
 __global__ void __launch_bounds__((Geo<T, L>::NT))
     k_xr2c(const double* __restrict__ in, cpx<T>* __restrict__ out, R2CIO a,
-           const cpx<T>* __restrict__ tw) {
+           const cpx<T>* __restrict__ tw, CornerDiff cd) {
   constexpr int BX = Geo<T, L>::BX;
   using F = FftTile<T, L, -1, BX, true>;
   __shared__ cpx<T> lds[F::LDS];
   F f;
   const int64_t ra = 2 * (int64_t(blockIdx.x) * BX + f.b), rb = ra + 1;
   const bool oka = ra < a.nrows, okb = rb < a.nrows;
-  const double* pa = in + (oka ? (ra % a.ny_rows) * a.in_sy + (ra / a.ny_rows) * a.in_sz : 0);
-  const double* pb = in + (okb ? (rb % a.ny_rows) * a.in_sy + (rb / a.ny_rows) * a.in_sz : 0);
-  f.load([&](int, int p) {
-    T re = (oka && p < a.n_in) ? T(pa[p]) : T(0);
-    T im = (okb && p < a.n_in) ? T(pb[p]) : T(0);
-    return mk(re, im);
-  });
+  if constexpr (DIFF) {
+    bool ina, inb;
+    const double* pa = diff_row(in, a, cd, ra, oka, ina);
+    const double* pb = diff_row(in, a, cd, rb, okb, inb);
+    f.load([&](int, int p) {
+      T re = (oka && p < a.n_in) ? T(diff_elem(pa, a, cd, ina, p)) : T(0);
+      T im = (okb && p < a.n_in) ? T(diff_elem(pb, a, cd, inb, p)) : T(0);
+      return mk(re, im);
+    });
+  } else {
+    const double* pa = in + (oka ? (ra % a.ny_rows) * a.in_sy + (ra / a.ny_rows) * a.in_sz : 0);
+    const double* pb = in + (okb ? (rb % a.ny_rows) * a.in_sy + (rb / a.ny_rows) * a.in_sz : 0);
+    f.load([&](int, int p) {
+      T re = (oka && p < a.n_in) ? T(pa[p]) : T(0);
+      T im = (okb && p < a.n_in) ? T(pb[p]) : T(0);
+      return mk(re, im);
+    });
+  }
   f.run(lds, tw);
   __syncthreads();
   f.dump(lds);

@@ -21,6 +21,10 @@ void launch_axis(int L, int dir, const cpx<T>* in, cpx<T>* out, const AxisIO& a,
 template <typename T>
 void launch_xr2c(int L, const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw,
                  hipStream_t s);
+// x pass over the corner difference of a primitive F (lengths with a plan only)
+template <typename T>
+void launch_xr2c_diff(int L, const double* F, cpx<T>* out, const R2CIO& a, const CornerDiff& cd,
+                      const cpx<T>* tw, hipStream_t s);
 template <typename T>
 void launch_xc2r(int L, const cpx<T>* in, const C2RIO<T>& a, int nout, const cpx<T>* tw,
                  hipStream_t s);

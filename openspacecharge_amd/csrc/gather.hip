@@ -166,12 +166,11 @@ __global__ void k_interp_rec(const double* __restrict__ R, int wantE, int wantB,
   __syncthreads();
   if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
 }
-// cell-ordered gather: slot s (particle sv[s], record srec[s]) leaves
-// (E, B, 0, 0) in the 64-byte record out8[s], written contiguously; the
-// values are those of k_interp_rec (same terms, same order)
+// cell-ordered gather: slot s (record srec[s]) leaves (E, B, clamped, 0) in
+// the 64-byte record out8[s], written contiguously; the values are those of
+// k_interp_rec (same terms, same order)
 __global__ void k_interp_cells(const double* __restrict__ R, int wantE, int wantB, int64_t n,
-                               const double4* __restrict__ srec, const uint32_t* __restrict__ sv,
-                               IP p, double* __restrict__ out8, int8_t* __restrict__ clamped,
+                               const double4* __restrict__ srec, IP p, double* __restrict__ out8,
                                uint32_t* __restrict__ nclamp) {
   __shared__ uint32_t cnt;
   if (threadIdx.x == 0) cnt = 0;
@@ -203,8 +202,7 @@ __global__ void k_interp_cells(const double* __restrict__ R, int wantE, int want
       for (int c = 0; c < 3; ++c) v[3 + c] = wsum8(R, sy, sz, b, 3 + c, ab, de, gh);
     double4* o = reinterpret_cast<double4*>(out8 + 8 * s);
     o[0] = make_double4(v[0], v[1], v[2], v[3]);
-    o[1] = make_double4(v[4], v[5], 0.0, 0.0);
-    if (clamped) clamped[sv[s]] = fl ? 1 : 0;
+    o[1] = make_double4(v[4], v[5], fl ? 1.0 : 0.0, 0.0);
     mine += fl ? 1u : 0u;
   }
   if (mine) atomicAdd(&cnt, mine);
@@ -213,10 +211,11 @@ __global__ void k_interp_cells(const double* __restrict__ R, int wantE, int want
 }
 
 // original order: particle i reads its slot inv[i] (one 64-byte record) and
-// writes E(3,n) and B(3,n) contiguously -- a random read costs one line,
-// while a random 24-byte write costs two partial sectors
+// writes E(3,n), B(3,n) and its clamp flag contiguously -- a random read costs
+// one line, while a random 24-byte write costs two partial sectors
 __global__ void k_unpermute(const double* __restrict__ out8, const uint32_t* __restrict__ inv,
-                            int64_t n, double* __restrict__ E, double* __restrict__ B) {
+                            int64_t n, double* __restrict__ E, double* __restrict__ B,
+                            int8_t* __restrict__ clamped) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
     const double4* r = reinterpret_cast<const double4*>(out8 + 8 * int64_t(inv[i]));
@@ -231,6 +230,7 @@ __global__ void k_unpermute(const double* __restrict__ out8, const uint32_t* __r
       B[3 * i + 1] = b.x;
       B[3 * i + 2] = b.y;
     }
+    if (clamped) clamped[i] = b.z != 0.0 ? 1 : 0;
   }
 }
 }  // namespace
@@ -266,14 +266,13 @@ void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const
       ProfScope ps(c, ST_GATHER);
       hipLaunchKernelGGL(k_interp_cells, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
                          rec.as<const double>(), int(E && ef), int(B && bf), n,
-                         sp.srec.as<const double4>(), sp.sv.as<const uint32_t>(), p,
-                         out8.as<double>(), clamped, cnt);
+                         sp.srec.as<const double4>(), p, out8.as<double>(), cnt);
       OSC_HIP(hipGetLastError());
     }
     ProfScope po(c, ST_GATHER_OUT);
     hipLaunchKernelGGL(k_unpermute, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
                        out8.as<const double>(), sp.inv.as<const uint32_t>(), n, (E && ef) ? E : nullptr,
-                       (B && bf) ? B : nullptr);
+                       (B && bf) ? B : nullptr, clamped);
     OSC_HIP(hipGetLastError());
   }
 }

@@ -184,35 +184,41 @@ __global__ void k_convert(const cpx<double>* __restrict__ in, cpx<float>* __rest
 
 }  // namespace
 
+// primitive F of an offset-kind Green function (open_spacecharge_mod.f90:662-691)
+static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F) {
+  const double dx = s.delta[0], dy = s.delta[1], dz = s.delta[2] * s.gamma;
+  const double factor =
+      (s.icomp == 1 || s.icomp == 2) ? s.gamma / (dx * dy * dz) : 1.0 / (dx * dy * dz);
+  double umin = (0.5 - double(s.L[0] / 2)) * dx;
+  double vmin = (0.5 - double(s.L[1] / 2)) * dy;
+  double wmin = (0.5 - double(s.L[2] / 2)) * dz;
+  umin = umin + s.offset[0];
+  vmin = vmin + s.offset[1];
+  wmin = wmin + s.offset[2] * s.gamma;
+  const dim3 g3(unsigned((s.L[0] + 63) / 64), unsigned(s.L[1]), unsigned(s.L[2]));
+  // the solve path evaluates E components on half the grid along their own
+  // axis (mirror symmetry, see k_prim_offset); the returned-to-caller Green
+  // function (zero_raw == 0) keeps upstream's coordinate arithmetic
+  int mir = -1;
+  if (s.zero_raw && s.icomp >= 1 && s.offset[s.icomp - 1] == 0.0 && s.L[s.icomp - 1] % 2 == 0)
+    mir = s.icomp - 1;
+  dim3 gp = g3;
+  if (mir == 0) gp.x = unsigned((s.L[0] / 2 + 63) / 64);
+  if (mir == 1) gp.y = unsigned(s.L[1] / 2);
+  if (mir == 2) gp.z = unsigned(s.L[2] / 2);
+  hipLaunchKernelGGL(k_prim_offset, gp, dim3(64), 0, c->stream, F, s.L[0], s.L[1], s.L[2], dx, dy,
+                     dz, umin, vmin, wmin, factor, s.icomp, mir);
+  OSC_HIP(hipGetLastError());
+}
+
 void build_green_real(osc_ctx* c, const GreenSpec& s, double* g) {
   const int64_t N = int64_t(s.L[0]) * s.L[1] * s.L[2];
   ProfScope ps(c, ST_GREEN_EVAL);
   if (s.kind == GK_OFFSET) {
     // osc_get_cgrn_freespace (open_spacecharge_mod.f90:633-707)
-    const double dx = s.delta[0], dy = s.delta[1], dz = s.delta[2] * s.gamma;
-    const double factor =
-        (s.icomp == 1 || s.icomp == 2) ? s.gamma / (dx * dy * dz) : 1.0 / (dx * dy * dz);
-    double umin = (0.5 - double(s.L[0] / 2)) * dx;
-    double vmin = (0.5 - double(s.L[1] / 2)) * dy;
-    double wmin = (0.5 - double(s.L[2] / 2)) * dz;
-    umin = umin + s.offset[0];
-    vmin = vmin + s.offset[1];
-    wmin = wmin + s.offset[2] * s.gamma;
     DevBuf F(c->pool, sizeof(double) * N);
+    build_offset_primitive(c, s, F.as<double>());
     const dim3 g3(unsigned((s.L[0] + 63) / 64), unsigned(s.L[1]), unsigned(s.L[2]));
-    // the solve path evaluates E components on half the grid along their own
-    // axis (mirror symmetry, see k_prim_offset); the returned-to-caller Green
-    // function (zero_raw == 0) keeps upstream's coordinate arithmetic
-    int mir = -1;
-    if (s.zero_raw && s.icomp >= 1 && s.offset[s.icomp - 1] == 0.0 && s.L[s.icomp - 1] % 2 == 0)
-      mir = s.icomp - 1;
-    dim3 gp = g3;
-    if (mir == 0) gp.x = unsigned((s.L[0] / 2 + 63) / 64);
-    if (mir == 1) gp.y = unsigned(s.L[1] / 2);
-    if (mir == 2) gp.z = unsigned(s.L[2] / 2);
-    hipLaunchKernelGGL(k_prim_offset, gp, dim3(64), 0, c->stream, F.as<double>(), s.L[0], s.L[1],
-                       s.L[2], dx, dy, dz, umin, vmin, wmin, factor, s.icomp, mir);
-    OSC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_diff_offset, g3, dim3(64), 0, c->stream, F.as<const double>(), g, s.L[0],
                        s.L[1], s.L[2], s.out_shift_z, s.zero_raw);
     OSC_HIP(hipGetLastError());
@@ -237,7 +243,8 @@ void build_green_real(osc_ctx* c, const GreenSpec& s, double* g) {
 }
 
 template <typename T>
-void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const SpecDims& d) {
+void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const SpecDims& d,
+                           const CornerDiff* cd) {
   const int L0 = d.L[0], L1 = d.L[1], L2 = d.L[2];
   ProfScope ps(c, ST_GREEN_FFT);
   R2CIO r{};
@@ -248,7 +255,10 @@ void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const Spec
   r.nrows = int64_t(L1) * L2;
   r.ny_rows = L1;
   r.n_in = L0;
-  launch_xr2c<T>(L0, g, spec, r, twiddles<T>(c, L0), c->stream);
+  if (cd)
+    launch_xr2c_diff<T>(L0, g, spec, r, *cd, twiddles<T>(c, L0), c->stream);
+  else
+    launch_xr2c<T>(L0, g, spec, r, twiddles<T>(c, L0), c->stream);
   AxisIO ay{};
   ay.s_b = ay.d_b = 1;
   ay.s_p = ay.d_p = d.KXP;
@@ -268,8 +278,10 @@ void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const Spec
   az.n_out = L2;
   launch_axis<T>(L2, -1, spec, spec, az, L1, 1, twiddles<T>(c, L2), c->stream);
 }
-template void spectrum_forward_full<double>(osc_ctx*, const double*, cpx<double>*, const SpecDims&);
-template void spectrum_forward_full<float>(osc_ctx*, const double*, cpx<float>*, const SpecDims&);
+template void spectrum_forward_full<double>(osc_ctx*, const double*, cpx<double>*, const SpecDims&,
+                                            const CornerDiff*);
+template void spectrum_forward_full<float>(osc_ctx*, const double*, cpx<float>*, const SpecDims&,
+                                           const CornerDiff*);
 
 void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_t n) {
   ProfScope ps(c, ST_GREEN_FFT);
@@ -302,9 +314,20 @@ const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) 
     ++gc.misses;
   }
   DevBuf g(c->pool, sizeof(double) * N);
-  build_green_real(c, s, g.as<double>());
   DevBuf sp(c->pool, sizeof(cpx<T>) * d.size());
-  spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d);
+  if (s.kind == GK_OFFSET && fft_has_plan(s.L[0])) {
+    // the x pass takes the 8-corner difference of the primitive as it loads
+    // its rows, so the difference array is never stored
+    {
+      ProfScope ps(c, ST_GREEN_EVAL);
+      build_offset_primitive(c, s, g.as<double>());
+    }
+    const CornerDiff cd{s.L[2], s.out_shift_z, s.zero_raw, 0};
+    spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d, &cd);
+  } else {
+    build_green_real(c, s, g.as<double>());
+    spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d);
+  }
   g.reset();
   if (!gc.enable) {
     keepalive = std::move(sp);

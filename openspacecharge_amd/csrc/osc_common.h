@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -89,7 +90,7 @@ class DevPool {
   ~DevPool() { release_all(); }
 
  private:
-  std::unordered_multimap<size_t, void*> free_;
+  std::multimap<size_t, void*> free_;  // by size: best fit
   std::unordered_map<void*, size_t> live_;
   size_t cached_ = 0;
 };

@@ -318,19 +318,24 @@ void space_charge_3d(osc_ctx* c, const osc_mesh& m, const double* rho, const dou
   if (c->prec == OSC_PREC_FP32) sc3d_t<float>(c, m, rho, offset, at_cathode, calc_b, ef, bf);
   else sc3d_t<double>(c, m, rho, offset, at_cathode, calc_b, ef, bf);
 }
+// Solves that take E as the finite-difference gradient of phi (legacy with
+// direct_field_calc = 0, the rectangular pipe) run the convolution in FP64
+// in either precision mode: the differences of neighbouring phi values
+// cancel, amplifying phi's relative rounding by about the bunch size in
+// cells, which in FP32 exceeds the 1e-5 field tolerance.
 int legacy_free(osc_ctx* c, const osc_mesh& m, const double* rho, int direct, int igf,
                 double* phi, double* ef, double* bf) {
-  if (c->prec == OSC_PREC_FP32) return legacy_free_t<float>(c, m, rho, direct, igf, phi, ef, bf);
+  if (c->prec == OSC_PREC_FP32 && direct) return legacy_free_t<float>(c, m, rho, direct, igf, phi, ef, bf);
   return legacy_free_t<double>(c, m, rho, direct, igf, phi, ef, bf);
 }
 int cathode_legacy(osc_ctx* c, const osc_mesh& m, double* rho, int direct, int igf, int method,
                    double* phi, double* ef, double* bf) {
-  if (c->prec == OSC_PREC_FP32) return cathode_t<float>(c, m, rho, direct, igf, method, phi, ef, bf);
+  if (c->prec == OSC_PREC_FP32 && direct)
+    return cathode_t<float>(c, m, rho, direct, igf, method, phi, ef, bf);
   return cathode_t<double>(c, m, rho, direct, igf, method, phi, ef, bf);
 }
 int rectpipe(osc_ctx* c, const osc_mesh& m, int direct, const double* rho, double* phi, double* ef,
              double* bf) {
-  if (c->prec == OSC_PREC_FP32) return pipe_t<float>(c, m, direct, rho, phi, ef, bf);
   return pipe_t<double>(c, m, direct, rho, phi, ef, bf);
 }
 
