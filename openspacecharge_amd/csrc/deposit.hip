@@ -390,9 +390,38 @@ __device__ __forceinline__ void cic_terms(double x, double y, double z, double q
   }
 }
 
+// DPP lane moves of the 64-wide wavefront (gfx9 family): row_shr:n shifts
+// within 16-lane rows, row_bcast:15 / row_bcast:31 pass lane 15 of a row / lane
+// 31 to the rows selected by the row mask.  Lanes with no source keep `old`.
+constexpr int kDppRowShr = 0x110, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t v) {
+  return uint32_t(__builtin_amdgcn_update_dpp(int(old), int(v), CTRL, ROWMASK, 0xF, false));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const uint64_t b = uint64_t(__double_as_longlong(v));
+  const uint32_t lo = dpp_u32<CTRL, ROWMASK>(0u, uint32_t(b));
+  const uint32_t hi = dpp_u32<CTRL, ROWMASK>(0u, uint32_t(b >> 32));
+  return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+// one step of the segmented scan: add the source lane's partial when it holds
+// the same key (keys are sorted, so equal keys are one contiguous run)
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void seg_step(uint32_t k, double (&t)[8]) {
+  const uint32_t kd = dpp_u32<CTRL, ROWMASK>(0xFFFFFFFFu, k);
+  double o[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) o[c] = dpp_f64<CTRL, ROWMASK>(t[c]);
+  if (kd == k)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) t[c] = o[c] + t[c];
+}
+
 // Each lane takes one sorted particle; a segmented inclusive scan over equal
-// keys (Hillis-Steele, fixed shuffle pattern) leaves each run's sum within
-// the wave at its last lane.  A run that starts and ends inside the wave is
+// keys (fixed DPP pattern: 16-lane rows, then row broadcasts) leaves each run's
+// sum within the wave at its last lane.  A run that starts and ends inside the wave is
 // written to S directly.  A run crossing a wave boundary leaves partials:
 // head[w] = its part in wave w when it began in an earlier wave and ends in w,
 // tail[w] = its part from its start in w (or lane 0) to lane 63 when it
@@ -425,16 +454,14 @@ __global__ void __launch_bounds__(kThreads)
       const double4 r = srec[i];
       cic_terms(r.x, r.y, r.z, p.hasq ? r.w : p.q_each, p, i0, j0, k0, t);
     }
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t kd = __shfl_up(k, d, 64);
-      double o[8];
-#pragma unroll
-      for (int c = 0; c < 8; ++c) o[c] = __shfl_up(t[c], d, 64);
-      if (lane >= d && kd == k)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) t[c] = o[c] + t[c];
-    }
+    // within 16-lane rows: shifts 1, 2, 4, 8; then lane 15 of rows 0 and 2
+    // into rows 1 and 3, then lane 31 into rows 2 and 3
+    seg_step<kDppRowShr + 1, 0xF>(k, t);
+    seg_step<kDppRowShr + 2, 0xF>(k, t);
+    seg_step<kDppRowShr + 4, 0xF>(k, t);
+    seg_step<kDppRowShr + 8, 0xF>(k, t);
+    seg_step<kDppRowBcast15, 0xA>(k, t);
+    seg_step<kDppRowBcast31, 0xC>(k, t);
     const uint32_t kn = __shfl_down(k, 1, 64);
     const uint32_t kfirst = __shfl(k, 0, 64);
     if (!valid || (lane < 63 && kn == k)) continue;  // not the last lane of its run
@@ -454,15 +481,31 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// first index in [lo, hi) with ks[index] > k (ks sorted)
-__device__ __forceinline__ int64_t upper_bound_u32(const uint32_t* __restrict__ ks, int64_t lo,
-                                                   int64_t hi, uint32_t k) {
-  while (lo < hi) {
+// Wave holding the last element of the run of key k that continues past
+// the end of wave w (ks sorted, nv valid keys): the first wave ww > w whose
+// last element is not k, or the last wave.  Galloping over wave ends, then
+// bisection, so a run of m waves costs O(log m) reads.
+__device__ __forceinline__ int64_t run_end_wave(const uint32_t* __restrict__ ks, int64_t nv,
+                                                int64_t w, uint32_t k) {
+  const int64_t nw = (nv + 63) >> 6;
+  auto cont = [&](int64_t ww) {  // the run continues past the end of wave ww
+    const int64_t e = (ww << 6) + 64;
+    return e < nv && ks[e] == k;
+  };
+  int64_t lo = w, step = 1;  // cont(lo) holds
+  int64_t hi = w + 1;
+  while (hi < nw && cont(hi)) {
+    lo = hi;
+    step <<= 1;
+    hi = lo + step;
+  }
+  if (hi > nw - 1) hi = nw - 1;  // !cont(hi) holds (the last wave never continues)
+  while (hi - lo > 1) {
     const int64_t mid = lo + ((hi - lo) >> 1);
-    if (ks[mid] <= k) lo = mid + 1;
+    if (cont(mid)) lo = mid;
     else hi = mid;
   }
-  return lo;
+  return hi;
 }
 
 // One thread per wave whose last run continues into the next wave and began
@@ -482,20 +525,30 @@ __global__ void k_seg_fix(const uint32_t* __restrict__ ks, int64_t n,
     const uint32_t k = ks[last];
     if (ks[last + 1] != k) continue;                          // no continuation
     if (ks[w << 6] == k && w > 0 && ks[(w << 6) - 1] == k) continue;  // began earlier
-    const int64_t e = upper_bound_u32(ks, last + 1, nv, k);   // one past the run
-    const int64_t w1 = (e - 1) >> 6;
-    if (w1 - w > kSegSerial) {
-      heavy[atomicAdd(nheavy, 1u)] = uint32_t(w);
-      continue;
-    }
+    // the middle waves are summed in order as their ends are checked; past
+    // kSegSerial of them the run goes to k_seg_heavy
     double acc[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) acc[c] = tail[w * 8 + c];
-    for (int64_t ww = w + 1; ww < w1; ++ww)
+    int64_t ww = w + 1;
+    bool long_run = false;
+    for (;;) {
+      const int64_t e = (ww << 6) + 64;  // first element of the next wave
+      if (!(e < nv && ks[e] == k)) break;  // the run ends in wave ww
+      if (ww + 1 - w > kSegSerial) {
+        long_run = true;
+        break;
+      }
 #pragma unroll
       for (int c = 0; c < 8; ++c) acc[c] = acc[c] + tail[ww * 8 + c];
+      ++ww;
+    }
+    if (long_run) {
+      heavy[atomicAdd(nheavy, 1u)] = uint32_t(w);
+      continue;
+    }
 #pragma unroll
-    for (int c = 0; c < 8; ++c) S[c * ncell + k] = acc[c] + head[w1 * 8 + c];
+    for (int c = 0; c < 8; ++c) S[c * ncell + k] = acc[c] + head[ww * 8 + c];
   }
 }
 
@@ -511,9 +564,8 @@ __global__ void __launch_bounds__(kThreads)
   const uint32_t nh = *nheavy;
   for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
     const int64_t w = heavy[h];
-    const int64_t last = (w << 6) + 63;
-    const uint32_t k = ks[last];
-    const int64_t w1 = (upper_bound_u32(ks, last + 1, nv, k) - 1) >> 6;
+    const uint32_t k = ks[(w << 6) + 63];
+    const int64_t w1 = run_end_wave(ks, nv, w, k);
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int64_t ww = w + 1 + threadIdx.x; ww < w1; ww += kThreads)
 #pragma unroll
