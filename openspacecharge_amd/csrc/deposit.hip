@@ -116,6 +116,11 @@ __global__ void k_sum_final(const double* __restrict__ part, int nb, double* __r
   if (threadIdx.x == 0) out[0] = r[0];
 }
 
+// tiles of the radix sort: RS_T threads, RS_I elements each, per block; the
+// key kernel covers one tile with KEY_T threads
+constexpr int RS_T = 256, RS_I = 16, RS_TILE = RS_T * RS_I, RS_W = RS_T / 64;
+constexpr int KEY_T = 1024, KEY_I = RS_TILE / KEY_T;
+
 // ---------------------------------------------------------------------------
 // step 1: cell keys
 // ---------------------------------------------------------------------------
@@ -148,17 +153,25 @@ __device__ __forceinline__ int gather_cell(double x, double mn, double dinv, int
 
 // cell key of every particle, and its 32-byte record (x, y, z, q) in input
 // order (the sort's payload: one record per particle instead of three
-// scattered coordinates)
-__global__ void k_cic_keys(const double* __restrict__ x, const double* __restrict__ y,
-                           const double* __restrict__ z, const double* __restrict__ q, int64_t n,
-                           KeyParams p, uint32_t* __restrict__ keys, double4* __restrict__ rec,
-                           uint32_t* __restrict__ ndrop) {
+// scattered coordinates).  Block b takes the sort tile b (RS_TILE particles)
+// and leaves the tile's count of each first-pass digit (key & mask0) in
+// hist[d * nb + b], the layout of k_rs_hist.
+__global__ void __launch_bounds__(KEY_T)
+    k_cic_keys(const double* __restrict__ x, const double* __restrict__ y,
+               const double* __restrict__ z, const double* __restrict__ q, int64_t n, KeyParams p,
+               uint32_t* __restrict__ keys, double4* __restrict__ rec,
+               uint32_t* __restrict__ ndrop, uint32_t mask0, uint32_t* __restrict__ hist, int nb) {
+  __shared__ uint32_t h[256];
   __shared__ uint32_t cnt;
+  for (int d = threadIdx.x; d < 256; d += KEY_T) h[d] = 0;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   uint32_t mine = 0;
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x) {
+  const int64_t base = int64_t(blockIdx.x) * RS_TILE;
+#pragma unroll
+  for (int it = 0; it < KEY_I; ++it) {
+    const int64_t i = base + it * KEY_T + threadIdx.x;
+    if (i >= n) break;
     const double xi = x[i], yi = y[i], zi = z[i];
     int i0, j0, k0;
     bool ok = true;
@@ -171,21 +184,23 @@ __global__ void k_cic_keys(const double* __restrict__ x, const double* __restric
            cell_of(yi, p.mn[1], p.dinv[1], p.base[1], p.nlo[1], p.nn[1], j0) &&
            cell_of(zi, p.mn[2], p.dinv[2], p.base[2], p.nlo[2], p.nn[2], k0);
     }
-    keys[i] = ok ? uint32_t(i0) + uint32_t(p.nn[0]) * (uint32_t(j0) + uint32_t(p.nn[1]) * uint32_t(k0))
-                 : p.sentinel;
+    const uint32_t key =
+        ok ? uint32_t(i0) + uint32_t(p.nn[0]) * (uint32_t(j0) + uint32_t(p.nn[1]) * uint32_t(k0))
+           : p.sentinel;
+    keys[i] = key;
     rec[i] = make_double4(xi, yi, zi, q ? q[i] : 0.0);
+    atomicAdd(&h[key & mask0], 1u);
     mine += ok ? 0u : 1u;
   }
   if (mine) atomicAdd(&cnt, mine);
   __syncthreads();
+  for (int d = threadIdx.x; d <= int(mask0); d += KEY_T) hist[int64_t(d) * nb + blockIdx.x] = h[d];
   if (threadIdx.x == 0 && cnt) atomicAdd(ndrop, cnt);
 }
 
 // ---------------------------------------------------------------------------
 // step 2: stable LSD radix sort of (key, value) pairs
 // ---------------------------------------------------------------------------
-constexpr int RS_T = 256, RS_I = 16, RS_TILE = RS_T * RS_I, RS_W = RS_T / 64;
-
 __global__ void __launch_bounds__(RS_T)
     k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift, uint32_t mask,
               uint32_t* __restrict__ hist, int nb) {
@@ -201,6 +216,36 @@ __global__ void __launch_bounds__(RS_T)
   for (int d = threadIdx.x; d <= int(mask); d += RS_T) hist[int64_t(d) * nb + blockIdx.x] = h[d];
 }
 
+// Per-block digit counts hist[d * nb + b] -> their exclusive scan over the
+// blocks of each digit (in place), and each digit's total.  One block per
+// digit; thread t scans a contiguous slice of the row.
+__global__ void __launch_bounds__(RS_T)
+    k_digit_scan(uint32_t* __restrict__ hist, int nb, uint32_t* __restrict__ total) {
+  __shared__ uint32_t s[RS_T];
+  uint32_t* row = hist + int64_t(blockIdx.x) * nb;
+  const int per = (nb + RS_T - 1) / RS_T;
+  const int b0 = int(threadIdx.x) * per;
+  uint32_t acc = 0;
+  for (int q = 0; q < per; ++q)
+    if (b0 + q < nb) acc += row[b0 + q];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 1; off < RS_T; off <<= 1) {
+    const uint32_t a = int(threadIdx.x) >= off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += a;
+    __syncthreads();
+  }
+  uint32_t run = s[threadIdx.x] - acc;
+  for (int q = 0; q < per; ++q)
+    if (b0 + q < nb) {
+      const uint32_t v = row[b0 + q];
+      row[b0 + q] = run;
+      run += v;
+    }
+  if (threadIdx.x == RS_T - 1) total[blockIdx.x] = s[RS_T - 1];
+}
+
 // payload gathered after the last pass: the particle records in sorted order
 struct SortPayload {
   const double4* rec;
@@ -208,41 +253,53 @@ struct SortPayload {
   uint32_t* inv;  // inv[original index] = sorted slot (may be null)
 };
 
+// inclusive scan of one value per thread over the RS_T threads: within each
+// wave by shuffles, then the wave totals (s holds RS_W entries)
+__device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t* s) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  if (lane == 63) s[wave] = v;
+  __syncthreads();
+  for (int w = 0; w < wave; ++w) v += s[w];
+  __syncthreads();
+  return v;
+}
+
 // One pass: each block ranks its tile of RS_TILE pairs by digit (stable: lane
 // order within a wave, wave order, then iteration order), stages the
 // tile in LDS in that order, and writes it out so that consecutive threads
-// store consecutive slots of one digit's run.  offs[d * nb + b] is the
-// exclusive scan (digit-major) of the per-block digit counts, so the count of
-// (d, b) is the difference to the next entry (n for the last one).
+// store consecutive slots of one digit's run.  col[d * nb + b] is the
+// exclusive scan over blocks of digit d's per-block counts (k_digit_scan) and
+// total[d] that digit's count; the slots of digit d start after all smaller
+// digits.
 __global__ void __launch_bounds__(RS_T)
     k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                  uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n, int shift,
-                 int bits, const uint32_t* __restrict__ offs, int nb) {
+                 int bits, const uint32_t* __restrict__ col, const uint32_t* __restrict__ total,
+                 int nb) {
   __shared__ uint32_t lk[RS_TILE], lv[RS_TILE];
   __shared__ uint32_t run[256], gofs[256], scan[256];
   __shared__ uint32_t wc[RS_W][256];
   const uint32_t mask = (1u << bits) - 1u;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
-  const int64_t nflat = int64_t(mask + 1) * nb;
   {
-    // this block's count of each digit, and their exclusive scan over digits
     const int d = threadIdx.x;  // RS_T == 256 digits at most
+    const uint32_t tot = d <= int(mask) ? total[d] : 0u;
+    const uint32_t dbase = block_scan_incl(tot, scan) - tot;  // first slot of digit d
+    // this block's count of each digit, and their exclusive scan over digits
     uint32_t cnt = 0, g = 0;
     if (d <= int(mask)) {
       const int64_t f = int64_t(d) * nb + blockIdx.x;
-      g = offs[f];
-      cnt = (f + 1 < nflat ? offs[f + 1] : uint32_t(n)) - g;
+      const uint32_t c0 = col[f];
+      cnt = (int(blockIdx.x) + 1 < nb ? col[f + 1] : tot) - c0;
+      g = dbase + c0;
     }
-    scan[d] = cnt;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-      const uint32_t a = d >= off ? scan[d - off] : 0u;
-      __syncthreads();
-      scan[d] += a;
-      __syncthreads();
-    }
-    const uint32_t start = scan[d] - cnt;  // exclusive
+    const uint32_t start = block_scan_incl(cnt, scan) - cnt;  // exclusive
     run[d] = start;
     gofs[d] = g - start;  // global slot of local slot j of digit d: gofs[d] + j
   }
@@ -296,49 +353,6 @@ __global__ void k_sort_payload(const uint32_t* __restrict__ sv, int64_t n, SortP
     const uint32_t v = sv[s];
     pl.srec[s] = pl.rec[v];
     if (pl.inv) pl.inv[v] = uint32_t(s);
-  }
-}
-
-// exclusive scan of u32 (3-phase, recursive on block totals)
-constexpr int SC_T = 256, SC_I = 16, SC_TILE = SC_T * SC_I;
-
-__global__ void __launch_bounds__(SC_T)
-    k_scan_block(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t n,
-                 uint32_t* __restrict__ tot) {
-  __shared__ uint32_t s[SC_T];
-  const int64_t base = int64_t(blockIdx.x) * SC_TILE + int64_t(threadIdx.x) * SC_I;
-  uint32_t v[SC_I];
-  uint32_t acc = 0;
-  for (int k = 0; k < SC_I; ++k) {
-    const int64_t i = base + k;
-    v[k] = i < n ? in[i] : 0u;
-    const uint32_t t = v[k];
-    v[k] = acc;
-    acc += t;
-  }
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  for (int off = 1; off < SC_T; off <<= 1) {
-    uint32_t a = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
-    __syncthreads();
-    s[threadIdx.x] += a;
-    __syncthreads();
-  }
-  const uint32_t excl = threadIdx.x ? s[threadIdx.x - 1] : 0u;
-  for (int k = 0; k < SC_I; ++k) {
-    const int64_t i = base + k;
-    if (i < n) out[i] = v[k] + excl;
-  }
-  if (threadIdx.x == SC_T - 1 && tot) tot[blockIdx.x] = s[SC_T - 1];
-}
-
-__global__ void k_scan_add(uint32_t* __restrict__ out, int64_t n,
-                           const uint32_t* __restrict__ tot) {
-  const int64_t i = blockIdx.x * int64_t(SC_TILE) + threadIdx.x;
-  const uint32_t a = tot[blockIdx.x];
-  for (int k = 0; k < SC_I; ++k) {
-    const int64_t j = i + int64_t(k) * SC_T;
-    if (j < n) out[j] += a;
   }
 }
 
@@ -431,27 +445,39 @@ __device__ __forceinline__ void seg_step(uint32_t k, double (&t)[8]) {
 // depends only on the sorted order: deterministic, no atomics on data.
 constexpr int kSegSerial = 64;
 
+//
+// Slot s holds particle sv[s]; its record rec[sv[s]] is read here (and, for a
+// following gather, stored to srec[s] with inv[sv[s]] = s) for every slot,
+// the dropped ones at the end included.
 __global__ void __launch_bounds__(kThreads)
-    k_seg_reduce(const uint32_t* __restrict__ ks, const double4* __restrict__ srec, int64_t n,
-                 const uint32_t* __restrict__ ndrop,
+    k_seg_reduce(const uint32_t* __restrict__ ks, const uint32_t* __restrict__ sv,
+                 const double4* __restrict__ rec, double4* __restrict__ srec,
+                 uint32_t* __restrict__ inv, int64_t n, const uint32_t* __restrict__ ndrop,
                  CellParams p, double* __restrict__ S, double* __restrict__ head,
                  double* __restrict__ tail) {
   const int64_t nv = n - int64_t(*ndrop);
   const int64_t ncell = int64_t(p.nn[0]) * p.nn[1] * p.nn[2];
   const int lane = threadIdx.x & 63;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  // the loop bound is uniform per block (blockDim is a multiple of 64)
-  for (int64_t base = int64_t(blockIdx.x) * blockDim.x; base < nv; base += stride) {
+  // the loop bound is uniform per block (blockDim is a multiple of 64), so
+  // every lane of a wave takes part in the DPP steps
+  for (int64_t base = int64_t(blockIdx.x) * blockDim.x; base < n; base += stride) {
     const int64_t i = base + threadIdx.x;
     const int64_t w = i >> 6;
     const bool valid = i < nv;
+    double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (i < n) {
+      const uint32_t v = sv[i];
+      r = rec[v];
+      if (srec) srec[i] = r;
+      if (inv) inv[v] = uint32_t(i);
+    }
     const uint32_t k = valid ? ks[i] : 0xFFFFFFFFu;
     double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (valid) {
       const int i0 = int(k % uint32_t(p.nn[0]));
       const int j0 = int((k / uint32_t(p.nn[0])) % uint32_t(p.nn[1]));
       const int k0 = int(k / (uint32_t(p.nn[0]) * uint32_t(p.nn[1])));
-      const double4 r = srec[i];
       cic_terms(r.x, r.y, r.z, p.hasq ? r.w : p.q_each, p, i0, j0, k0, t);
     }
     // within 16-lane rows: shifts 1, 2, 4, 8; then lane 15 of rows 0 and 2
@@ -645,53 +671,60 @@ double device_sum(osc_ctx* c, const double* v, int64_t n) {
   return r;
 }
 
-static void exclusive_scan(osc_ctx* c, const uint32_t* in, uint32_t* out, int64_t n) {
-  const int64_t nb = (n + SC_TILE - 1) / SC_TILE;
-  DevBuf tot(c->pool, sizeof(uint32_t) * (nb + 1));
-  hipLaunchKernelGGL(k_scan_block, dim3(unsigned(nb)), dim3(SC_T), 0, c->stream, in, out, n,
-                     tot.as<uint32_t>());
-  OSC_HIP(hipGetLastError());
-  if (nb > 1) {
-    DevBuf tsc(c->pool, sizeof(uint32_t) * (nb + 1));
-    exclusive_scan(c, tot.as<const uint32_t>(), tsc.as<uint32_t>(), nb);
-    hipLaunchKernelGGL(k_scan_add, dim3(unsigned(nb)), dim3(SC_T), 0, c->stream, out, n,
-                       tsc.as<const uint32_t>());
-    OSC_HIP(hipGetLastError());
-  }
-}
-
 void fill_u32(osc_ctx* c, uint32_t* p, int64_t n, uint32_t v) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill_u32, dim3(nblk(n)), dim3(kThreads), 0, c->stream, p, n, v);
   OSC_HIP(hipGetLastError());
 }
 
-// Sorts (key, value) pairs; values start as the input index.  The payload
-// (records, inverse permutation) is written from the sorted values at the
-// end.  When the key range needs no pass the pairs are already "sorted" and go
-// through a one-pass scatter with a zero-bit digit.
+// Digits of the LSD passes over nbits key bits: at most 8 bits per pass,
+// split evenly (a key range needing no bits still takes one zero-bit pass).
+struct SortPlan {
+  int npass = 1;
+  int bits[4] = {0, 0, 0, 0};
+  int nb = 0;  // tiles of RS_TILE
+  uint32_t mask(int p) const { return (1u << bits[p]) - 1u; }
+};
+static SortPlan sort_plan(int nbits, int64_t n) {
+  SortPlan sp;
+  sp.npass = nbits <= 0 ? 1 : (nbits + 7) / 8;
+  int shift = 0;
+  for (int p = 0; p < sp.npass; ++p) {
+    sp.bits[p] = nbits > 0 ? (nbits - shift + (sp.npass - p) - 1) / (sp.npass - p) : 0;
+    shift += sp.bits[p];
+  }
+  sp.nb = int((n + RS_TILE - 1) / RS_TILE);
+  return sp;
+}
+
+// Sorts (key, value) pairs; values start as the input index.  hist holds
+// 256 * nb counters; on entry it already holds the first pass's per-tile digit
+// counts (k_cic_keys).  With pl.srec the payload (records, inverse
+// permutation) is written from the sorted values at the end.
 static void radix_sort(osc_ctx* c, DevBuf& ka, DevBuf& va, DevBuf& kb, DevBuf& vb, int64_t n,
-                       int nbits, const SortPayload& pl, bool& result_in_b) {
-  int npass = nbits <= 0 ? 0 : (nbits + 7) / 8;
-  if (npass == 0) npass = 1;
-  const int nb = int((n + RS_TILE - 1) / RS_TILE);
+                       const SortPlan& sp, DevBuf& hist, const SortPayload& pl,
+                       bool& result_in_b) {
+  const int nb = sp.nb;
   result_in_b = false;
   if (n == 0) return;
-  DevBuf hist(c->pool, sizeof(uint32_t) * 256 * size_t(nb));
-  DevBuf offs(c->pool, sizeof(uint32_t) * 256 * size_t(nb));
+  DevBuf total(c->pool, sizeof(uint32_t) * 256);
   uint32_t *kin = ka.as<uint32_t>(), *vin = va.as<uint32_t>();
   uint32_t *kout = kb.as<uint32_t>(), *vout = vb.as<uint32_t>();
   int shift = 0;
-  for (int p = 0; p < npass; ++p) {
-    const int bits = nbits > 0 ? (nbits - shift + (npass - p) - 1) / (npass - p) : 0;
-    const uint32_t mask = (1u << bits) - 1u;
-    hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_T), 0, c->stream, kin, n, shift, mask,
-                       hist.as<uint32_t>(), nb);
+  for (int p = 0; p < sp.npass; ++p) {
+    const int bits = sp.bits[p];
+    const uint32_t mask = sp.mask(p);
+    if (p > 0) {
+      hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_T), 0, c->stream, kin, n, shift, mask,
+                         hist.as<uint32_t>(), nb);
+      OSC_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_digit_scan, dim3(mask + 1), dim3(RS_T), 0, c->stream, hist.as<uint32_t>(),
+                       nb, total.as<uint32_t>());
     OSC_HIP(hipGetLastError());
-    exclusive_scan(c, hist.as<const uint32_t>(), offs.as<uint32_t>(), int64_t(mask + 1) * nb);
     hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_T), 0, c->stream, kin,
                        p == 0 ? nullptr : vin, kout, vout, n, shift, bits,
-                       offs.as<const uint32_t>(), nb);
+                       hist.as<const uint32_t>(), total.as<const uint32_t>(), nb);
     OSC_HIP(hipGetLastError());
     std::swap(kin, kout);
     std::swap(vin, vout);
@@ -723,7 +756,8 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   const int64_t n = a.n;
   DevBuf ka(c->pool, sizeof(uint32_t) * (n + 1)), va(c->pool, sizeof(uint32_t) * (n + 1));
   DevBuf kb(c->pool, sizeof(uint32_t) * (n + 1)), vb(c->pool, sizeof(uint32_t) * (n + 1));
-  DevBuf rec(c->pool, sizeof(double4) * (n + 1)), srec(c->pool, sizeof(double4) * (n + 1));
+  DevBuf rec(c->pool, sizeof(double4) * (n + 1));
+  DevBuf srec(c->pool, keep ? sizeof(double4) * (n + 1) : 0);
   KeyParams kp{};
   CellParams cp{};
   for (int d = 0; d < 3; ++d) {
@@ -742,23 +776,26 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   cp.hasq = a.q != nullptr;
   cp.harness = a.harness;
   cp.q_each = a.q_each;
-  if (n > 0) {
-    ProfScope ps(c, ST_DEP_KEYS);
-    hipLaunchKernelGGL(k_cic_keys, dim3(nblk(n)), dim3(kThreads), 0, c->stream, a.x, a.y, a.z,
-                       a.q, n, kp, ka.as<uint32_t>(), rec.as<double4>(), cnt);
-    OSC_HIP(hipGetLastError());
-  }
   int nbits = 0;
   while ((uint64_t(1) << nbits) <= uint64_t(ncell)) ++nbits;
+  const SortPlan sp = sort_plan(nbits, n);
+  DevBuf hist(c->pool, sizeof(uint32_t) * 256 * size_t(sp.nb + 1));
+  if (n > 0) {
+    ProfScope ps(c, ST_DEP_KEYS);
+    hipLaunchKernelGGL(k_cic_keys, dim3(sp.nb), dim3(KEY_T), 0, c->stream, a.x, a.y, a.z, a.q, n, kp,
+                       ka.as<uint32_t>(), rec.as<double4>(), cnt, sp.mask(0), hist.as<uint32_t>(),
+                       sp.nb);
+    OSC_HIP(hipGetLastError());
+  }
   bool in_b = false;
   DevBuf inv(c->pool, keep ? sizeof(uint32_t) * (n + 1) : 0);
-  SortPayload pl{rec.as<const double4>(), srec.as<double4>(), keep ? inv.as<uint32_t>() : nullptr};
   {
     ProfScope ps(c, ST_DEP_SORT);
-    radix_sort(c, ka, va, kb, vb, n, nbits, pl, in_b);
+    radix_sort(c, ka, va, kb, vb, n, sp, hist, SortPayload{}, in_b);
   }
   ProfScope ps_cells(c, ST_DEP_CELLS);
   const uint32_t* sk = in_b ? kb.as<uint32_t>() : ka.as<uint32_t>();
+  const uint32_t* sv = in_b ? vb.as<uint32_t>() : va.as<uint32_t>();
   // corner sums of every cell (zero where no particle), wave partials of runs
   // crossing wave boundaries, list of long runs
   const int64_t nw = (n + 63) / 64 + 1;
@@ -766,9 +803,10 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
       tail(c->pool, sizeof(double) * 8 * nw), heavy(c->pool, sizeof(uint32_t) * nw);
   fill_f64(c, S.as<double>(), 8 * ncell, 0.0);
   if (n > 0) {
-    hipLaunchKernelGGL(k_seg_reduce, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk,
-                       srec.as<const double4>(), n, cnt, cp, S.as<double>(), head.as<double>(),
-                       tail.as<double>());
+    hipLaunchKernelGGL(k_seg_reduce, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk, sv,
+                       rec.as<const double4>(), keep ? srec.as<double4>() : nullptr,
+                       keep ? inv.as<uint32_t>() : nullptr, n, cnt, cp, S.as<double>(),
+                       head.as<double>(), tail.as<double>());
     OSC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_seg_fix, dim3(nblk(nw)), dim3(kThreads), 0, c->stream, sk, n, cnt, ncell,
                        head.as<const double>(), tail.as<const double>(), S.as<double>(),
@@ -814,18 +852,20 @@ void sort_for_gather(osc_ctx* c, const InterpArgs& a, int64_t n, const double* x
   kp.sentinel = uint32_t(ncell);
   uint32_t* cnt = static_cast<uint32_t*>(c->counters) + 3;  // unused drop count of the clamp rule
   fill_u32(c, cnt, 1, 0u);
-  if (n > 0) {
-    hipLaunchKernelGGL(k_cic_keys, dim3(nblk(n)), dim3(kThreads), 0, c->stream, x, y, z,
-                       static_cast<const double*>(nullptr), n, kp, ka.as<uint32_t>(),
-                       rec.as<double4>(), cnt);
-    OSC_HIP(hipGetLastError());
-  }
   int nbits = 0;
   while ((uint64_t(1) << nbits) <= uint64_t(ncell)) ++nbits;
+  const SortPlan sp = sort_plan(nbits, n);
+  DevBuf hist(c->pool, sizeof(uint32_t) * 256 * size_t(sp.nb + 1));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_cic_keys, dim3(sp.nb), dim3(KEY_T), 0, c->stream, x, y, z,
+                       static_cast<const double*>(nullptr), n, kp, ka.as<uint32_t>(),
+                       rec.as<double4>(), cnt, sp.mask(0), hist.as<uint32_t>(), sp.nb);
+    OSC_HIP(hipGetLastError());
+  }
   bool in_b = false;
   DevBuf inv(c->pool, sizeof(uint32_t) * (n + 1));
   SortPayload pl{rec.as<const double4>(), srec.as<double4>(), inv.as<uint32_t>()};
-  radix_sort(c, ka, va, kb, vb, n, nbits, pl, in_b);
+  radix_sort(c, ka, va, kb, vb, n, sp, hist, pl, in_b);
   out->n = n;
   out->inv = std::move(inv);
   out->sv = std::move(in_b ? vb : va);

@@ -17,30 +17,47 @@ inline int64_t nblk(int64_t n, int t = kThreads) {
 // primitive of osc_get_cgrn_freespace on the offset grid
 // (open_spacecharge_mod.f90:674-691): F = prim(u,v,w) * factor.
 // Grid (ceil(L0/64), L1, L2) of 64-thread blocks: no index division.
-// mir >= 0: the primitive is even in that axis (xlafun2 in its first
-// argument: x atan(yz/(rx)) is even in x, r depends on x^2) and the axis has no
-// offset, so its coordinate is taken as (i + 1/2 - L/2) * d -- a half-integer
-// times d, rounded once, so that the mirror point L-1-i is exactly its
-// negative -- and only the lower half along it is evaluated; each value is
-// stored at i and L-1-i.
+// Mirrored axes (the axis has no offset and even L): its coordinate is taken
+// as (i + 1/2 - L/2) * d -- a half-integer times d, rounded once, so that the
+// mirror point L-1-i is exactly its negative -- and only the lower half along
+// it is evaluated.  xlafun2 is even in its first argument (x atan(yz/(rx)) is
+// even in x, r depends on x^2): along `mev` the value is stored at i and
+// L-1-i.  It is odd in its third argument (every term changes sign with z):
+// along `mod` the negated value is stored at L-1-i; there the mirror half
+// differs from a direct evaluation only by the rounding of the quotient
+// inside the last logarithm.
 __global__ void __launch_bounds__(64)
     k_prim_offset(double* __restrict__ F, int L0, int L1, int L2, double dx, double dy, double dz,
-                  double umin, double vmin, double wmin, double factor, int icomp, int mir) {
+                  double umin, double vmin, double wmin, double factor, int icomp, int mev,
+                  int mod) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-  const int lim0 = mir == 0 ? L0 / 2 : L0;
-  if (i >= lim0) return;
-  const double w = mir == 2 ? (double(k) + (0.5 - double(L2 / 2))) * dz : double(k) * dz + wmin;
-  const double v = mir == 1 ? (double(j) + (0.5 - double(L1 / 2))) * dy : double(j) * dy + vmin;
-  const double u = mir == 0 ? (double(i) + (0.5 - double(L0 / 2))) * dx : double(i) * dx + umin;
+  const bool m0 = mev == 0 || mod == 0, m1 = mev == 1 || mod == 1, m2 = mev == 2 || mod == 2;
+  if (i >= (m0 ? L0 / 2 : L0)) return;
+  const double w = m2 ? (double(k) + (0.5 - double(L2 / 2))) * dz : double(k) * dz + wmin;
+  const double v = m1 ? (double(j) + (0.5 - double(L1 / 2))) * dy : double(j) * dy + vmin;
+  const double u = m0 ? (double(i) + (0.5 - double(L0 / 2))) * dx : double(i) * dx + umin;
   double g;
   if (icomp == 0) g = lafun2(u, v, w) * factor;
   else if (icomp == 1) g = xlafun2(u, v, w) * factor;
   else if (icomp == 2) g = xlafun2(v, w, u) * factor;
   else g = xlafun2(w, u, v) * factor;
-  F[i + int64_t(L0) * (j + int64_t(L1) * k)] = g;
-  if (mir == 0) F[(L0 - 1 - i) + int64_t(L0) * (j + int64_t(L1) * k)] = g;
-  if (mir == 1) F[i + int64_t(L0) * ((L1 - 1 - j) + int64_t(L1) * k)] = g;
-  if (mir == 2) F[i + int64_t(L0) * (j + int64_t(L1) * (L2 - 1 - k))] = g;
+  const int L[3] = {L0, L1, L2};
+  const int64_t sy = L0, sz = int64_t(L0) * L1;
+  const int64_t at = i + sy * j + sz * k;
+  F[at] = g;
+  int64_t ev = -1, od = -1;  // the mirror index along mev / mod
+  if (mev >= 0) {
+    const int c = mev == 0 ? i : (mev == 1 ? j : k);
+    ev = at + (L[mev] - 1 - 2 * c) * (mev == 0 ? 1 : (mev == 1 ? sy : sz));
+    F[ev] = g;
+  }
+  if (mod >= 0) {
+    const int c = mod == 0 ? i : (mod == 1 ? j : k);
+    const int64_t step = (L[mod] - 1 - 2 * c) * (mod == 0 ? 1 : (mod == 1 ? sy : sz));
+    od = at + step;
+    F[od] = -g;
+    if (ev >= 0) F[ev + step] = -g;
+  }
 }
 
 // 8-corner difference over 1:size-1 with the last planes left raw
@@ -196,18 +213,23 @@ static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F) {
   vmin = vmin + s.offset[1];
   wmin = wmin + s.offset[2] * s.gamma;
   const dim3 g3(unsigned((s.L[0] + 63) / 64), unsigned(s.L[1]), unsigned(s.L[2]));
-  // the solve path evaluates E components on half the grid along their own
-  // axis (mirror symmetry, see k_prim_offset); the returned-to-caller Green
-  // function (zero_raw == 0) keeps upstream's coordinate arithmetic
-  int mir = -1;
-  if (s.zero_raw && s.icomp >= 1 && s.offset[s.icomp - 1] == 0.0 && s.L[s.icomp - 1] % 2 == 0)
-    mir = s.icomp - 1;
+  // the solve path evaluates E components on a quarter of the grid (even
+  // mirror along their own axis, odd mirror along the axis of xlafun2's third
+  // argument, see k_prim_offset); the returned-to-caller Green function
+  // (zero_raw == 0) keeps upstream's coordinate arithmetic
+  auto mirrorable = [&](int ax) { return s.offset[ax] == 0.0 && s.L[ax] % 2 == 0; };
+  int mev = -1, mod = -1;
+  if (s.zero_raw && s.icomp >= 1) {
+    const int ea = s.icomp - 1, oa = (s.icomp + 1) % 3;  // first / third argument of xlafun2
+    if (mirrorable(ea)) mev = ea;
+    if (mirrorable(oa)) mod = oa;
+  }
   dim3 gp = g3;
-  if (mir == 0) gp.x = unsigned((s.L[0] / 2 + 63) / 64);
-  if (mir == 1) gp.y = unsigned(s.L[1] / 2);
-  if (mir == 2) gp.z = unsigned(s.L[2] / 2);
+  if (mev == 0 || mod == 0) gp.x = unsigned((s.L[0] / 2 + 63) / 64);
+  if (mev == 1 || mod == 1) gp.y = unsigned(s.L[1] / 2);
+  if (mev == 2 || mod == 2) gp.z = unsigned(s.L[2] / 2);
   hipLaunchKernelGGL(k_prim_offset, gp, dim3(64), 0, c->stream, F, s.L[0], s.L[1], s.L[2], dx, dy,
-                     dz, umin, vmin, wmin, factor, s.icomp, mir);
+                     dz, umin, vmin, wmin, factor, s.icomp, mev, mod);
   OSC_HIP(hipGetLastError());
 }
 
