@@ -19,10 +19,12 @@ void axis_t(int dir, const cpx<T>* in, cpx<T>* out, const AxisIO& a, int g1, int
             const cpx<T>* tw, hipStream_t s) {
   constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
   dim3 g = grid3(cdiv(a.nb, BX), g1, g2);
-  if (dir < 0)
-    hipLaunchKernelGGL((k_axis<T, L, -1>), g, dim3(NT), 0, s, in, out, a, tw);
+  if (dir < 0 && a.mh > 0)
+    hipLaunchKernelGGL((k_axis<T, L, -1, true>), g, dim3(NT), 0, s, in, out, a, tw);
+  else if (dir < 0)
+    hipLaunchKernelGGL((k_axis<T, L, -1, false>), g, dim3(NT), 0, s, in, out, a, tw);
   else
-    hipLaunchKernelGGL((k_axis<T, L, 1>), g, dim3(NT), 0, s, in, out, a, tw);
+    hipLaunchKernelGGL((k_axis<T, L, 1, false>), g, dim3(NT), 0, s, in, out, a, tw);
 }
 template <int L>
 void xr2c_t(const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw, hipStream_t s,
@@ -52,6 +54,8 @@ void zinv_t(const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, int Ly, int nout
 template <>
 void launch_axis<T>(int L, int dir, const cpx<T>* in, cpx<T>* out, const AxisIO& a, int g1, int g2,
                     const cpx<T>* tw, hipStream_t s) {
+  OSC_REQUIRE(a.mh == 0 || (dir < 0 && 2 * a.mh == L), OSC_ERR_ARG,
+              "half-stored input lines are for forward passes over even lengths");
   switch (L) {
 #define OSC_CASE(LL)                           \
   case LL:                                     \

@@ -20,7 +20,23 @@ struct AxisIO {
   int64_t s_b, s_p, s_o1, s_o2;
   int64_t d_b, d_p, d_o1, d_o2;
   int nb, n_in, w0, n_out;
+  // mh > 0: only elements [0, mh) of the input line are stored (mh = L/2), the
+  // line being mirror-symmetric about L/2 - 1: element p in [mh, L-2] is
+  // ms * element L-2-p and element L-1 is zero
+  int mh, ms;
 };
+
+// element p of an axis input line (see AxisIO::mh)
+template <typename T>
+__device__ __forceinline__ cpx<T> axis_in(const cpx<T>* ib, const AxisIO& a, int L, int p) {
+  if (p >= a.n_in) return mk<T>(0, 0);
+  if (a.mh > 0 && p >= a.mh) {
+    if (p == L - 1) return mk<T>(0, 0);
+    const cpx<T> v = ib[(L - 2 - p) * a.s_p];
+    return a.ms < 0 ? mk<T>(-v.x, -v.y) : v;
+  }
+  return ib[p * a.s_p];
+}
 
 // Real rows -> half spectra.  Row r (0 <= r < nrows) is (y, z) = (r % ny_rows,
 // r / ny_rows); input row at in + y*in_sy + z*in_sz (n_in reals, rest zero),
@@ -109,7 +125,8 @@ __device__ __forceinline__ cpx<T> ldc(const cpx<T>* p) {
 // ---------------------------------------------------------------------------
 // strided line FFT (y or z axis, or x with s_p = 1)
 // ---------------------------------------------------------------------------
-template <typename T, int L, int DIR>
+// MIR: the input lines are stored as halves (AxisIO::mh)
+template <typename T, int L, int DIR, bool MIR>
 __global__ void __launch_bounds__((Geo<T, L>::NT))
     k_axis(const cpx<T>* __restrict__ in, cpx<T>* out, AxisIO a, const cpx<T>* __restrict__ tw) {
   constexpr int BX = Geo<T, L>::BX;
@@ -119,7 +136,10 @@ __global__ void __launch_bounds__((Geo<T, L>::NT))
   const int bg = blockIdx.x * BX + f.b;
   const bool ok = bg < a.nb;
   const cpx<T>* ib = in + (ok ? bg * a.s_b + blockIdx.y * a.s_o1 + blockIdx.z * a.s_o2 : 0);
-  f.load([&](int, int p) { return (ok && p < a.n_in) ? ib[p * a.s_p] : mk<T>(0, 0); });
+  if constexpr (MIR)
+    f.load([&](int, int p) { return ok ? axis_in(ib, a, L, p) : mk<T>(0, 0); });
+  else
+    f.load([&](int, int p) { return (ok && p < a.n_in) ? ib[p * a.s_p] : mk<T>(0, 0); });
   f.run(lds, tw);
   cpx<T>* ob = out + (ok ? bg * a.d_b + blockIdx.y * a.d_o1 + blockIdx.z * a.d_o2 : 0);
   f.store([&](int, int p, cpx<T> v) {
@@ -138,7 +158,7 @@ __global__ void __launch_bounds__(kGenBX* kGenTPL)
   const int bg = blockIdx.x * kGenBX + f.b;
   const bool ok = bg < a.nb;
   const cpx<T>* ib = in + (ok ? bg * a.s_b + blockIdx.y * a.s_o1 + blockIdx.z * a.s_o2 : 0);
-  f.load(lds, [&](int, int p) { return (ok && p < a.n_in) ? ib[p * a.s_p] : mk<T>(0, 0); });
+  f.load(lds, [&](int, int p) { return ok ? axis_in(ib, a, pl.L, p) : mk<T>(0, 0); });
   f.run(pl, tw);
   cpx<T>* ob = out + (ok ? bg * a.d_b + blockIdx.y * a.d_o1 + blockIdx.z * a.d_o2 : 0);
   f.store([&](int, int p, cpx<T> v) {
@@ -162,9 +182,11 @@ __device__ __forceinline__ void r2c_unpack(cpx<T> zk, cpx<T> zm, cpx<T>& A, cpx<
 // F (open_spacecharge_mod.f90:697-705): row (j, kq) of the result is built
 // from rows (j, k), (j+1, k), (j, k+1), (j+1, k+1) of F with
 // k = (kq - sh) mod L2; elements on the last plane of any axis keep the raw
-// primitive, or 0 with zero_raw.  F has the strides of R2CIO's input.
+// primitive, or 0 with zero_raw.  F has the strides of R2CIO's input and
+// extents (n_in, L1, L2); R2CIO's rows may cover only the lower part in y
+// (ny_rows < L1) and z.
 struct CornerDiff {
-  int L2, sh, zero_raw, pad_;
+  int L1, L2, sh, zero_raw;
 };
 
 // row r of the corner-difference source: its F row and whether it is interior
@@ -177,7 +199,7 @@ __device__ __forceinline__ const double* diff_row(const double* F, const R2CIO& 
   int k = int(r / a.ny_rows) - cd.sh;
   k %= cd.L2;
   if (k < 0) k += cd.L2;
-  inner = j < a.ny_rows - 1 && k < cd.L2 - 1;
+  inner = j < cd.L1 - 1 && k < cd.L2 - 1;
   return F + j * a.in_sy + k * a.in_sz;
 }
 // element p of a corner-difference row, with the terms in the order of the

@@ -166,44 +166,82 @@ __global__ void k_interp_rec(const double* __restrict__ R, int wantE, int wantB,
   __syncthreads();
   if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
 }
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
 // cell-ordered gather: slot s (record srec[s]) leaves (E, B, clamped, 0) in
-// the 64-byte record out8[s], written contiguously; the values are those of
-// k_interp_rec (same terms, same order)
-__global__ void k_interp_cells(const double* __restrict__ R, int wantE, int wantB, int64_t n,
-                               const double4* __restrict__ srec, IP p, double* __restrict__ out8,
-                               uint32_t* __restrict__ nclamp) {
+// the 64-byte record out8[s]; the values are those of k_interp_rec (same
+// terms, same order).  Each wave stages its 64 records in LDS (9-double
+// stride) and stores them as 8 contiguous 512-byte runs: one 64-byte record
+// per lane would make every store instruction touch twice the lines.
+constexpr int kRecPad = 9;
+__global__ void __launch_bounds__(kThreads)
+    k_interp_cells(const double* __restrict__ R, int wantE, int wantB, int64_t n,
+                   const double4* __restrict__ srec, IP p, double* __restrict__ out8,
+                   uint32_t* __restrict__ nclamp) {
   __shared__ uint32_t cnt;
+  __shared__ double stage[kThreads / 64][64 * kRecPad];
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   uint32_t mine = 0;
+  const int lane = threadIdx.x & 63;
+  double* st = stage[threadIdx.x >> 6];
   const int64_t sy = p.nn[0], sz = int64_t(p.nn[0]) * p.nn[1];
-  for (int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; s < n;
-       s += int64_t(gridDim.x) * blockDim.x) {
-    const double4 r = srec[s];
-    const double xi = r.x, yi = r.y, zi = r.z;
-    bool fl = false;
-    const int ip = clamp_idx(xi, p.mn[0], p.hinv[0], p.nhi[0], fl);
-    const int jp = clamp_idx(yi, p.mn[1], p.hinv[1], p.nhi[1], fl);
-    const int kp = clamp_idx(zi, p.mn[2], p.hinv[2], p.nhi[2], fl);
-    const double ab = ((p.mn[0] - xi) + ip * p.delta[0]) * p.hinv[0];
-    const double de = ((p.mn[1] - yi) + jp * p.delta[1]) * p.hinv[1];
-    const double gh = ((p.mn[2] - zi) + kp * p.delta[2]) * p.hinv[2];
-    int i0 = ip - p.nlo[0], j0 = jp - p.nlo[1], k0 = kp - p.nlo[2];
-    i0 = i0 < 0 ? 0 : (i0 > p.nn[0] - 2 ? p.nn[0] - 2 : i0);
-    j0 = j0 < 0 ? 0 : (j0 > p.nn[1] - 2 ? p.nn[1] - 2 : j0);
-    k0 = k0 < 0 ? 0 : (k0 > p.nn[2] - 2 ? p.nn[2] - 2 : k0);
-    const int64_t b = i0 + j0 * sy + k0 * sz;
+  // wave-uniform loop over chunks of 64 slots
+  for (int64_t wb = blockIdx.x * int64_t(blockDim.x) + (threadIdx.x & ~63); wb < n;
+       wb += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t s = wb + lane;
     double v[6] = {0, 0, 0, 0, 0, 0};
-    if (wantE)
+    bool fl = false;
+    if (s < n) {
+      const double4 r = srec[s];
+      const double xi = r.x, yi = r.y, zi = r.z;
+      const int ip = clamp_idx(xi, p.mn[0], p.hinv[0], p.nhi[0], fl);
+      const int jp = clamp_idx(yi, p.mn[1], p.hinv[1], p.nhi[1], fl);
+      const int kp = clamp_idx(zi, p.mn[2], p.hinv[2], p.nhi[2], fl);
+      const double ab = ((p.mn[0] - xi) + ip * p.delta[0]) * p.hinv[0];
+      const double de = ((p.mn[1] - yi) + jp * p.delta[1]) * p.hinv[1];
+      const double gh = ((p.mn[2] - zi) + kp * p.delta[2]) * p.hinv[2];
+      int i0 = ip - p.nlo[0], j0 = jp - p.nlo[1], k0 = kp - p.nlo[2];
+      i0 = i0 < 0 ? 0 : (i0 > p.nn[0] - 2 ? p.nn[0] - 2 : i0);
+      j0 = j0 < 0 ? 0 : (j0 > p.nn[1] - 2 ? p.nn[1] - 2 : j0);
+      k0 = k0 < 0 ? 0 : (k0 > p.nn[2] - 2 ? p.nn[2] - 2 : k0);
+      const int64_t b = i0 + j0 * sy + k0 * sz;
+      if (wantE)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) v[c] = wsum8(R, sy, sz, b, c, ab, de, gh);
-    if (wantB)
+        for (int c = 0; c < 3; ++c) v[c] = wsum8(R, sy, sz, b, c, ab, de, gh);
+      if (wantB)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) v[3 + c] = wsum8(R, sy, sz, b, 3 + c, ab, de, gh);
-    double4* o = reinterpret_cast<double4*>(out8 + 8 * s);
-    o[0] = make_double4(v[0], v[1], v[2], v[3]);
-    o[1] = make_double4(v[4], v[5], fl ? 1.0 : 0.0, 0.0);
-    mine += fl ? 1u : 0u;
+        for (int c = 0; c < 3; ++c) v[3 + c] = wsum8(R, sy, sz, b, 3 + c, ab, de, gh);
+      mine += fl ? 1u : 0u;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) st[lane * kRecPad + c] = v[c];
+    st[lane * kRecPad + 6] = fl ? 1.0 : 0.0;
+    st[lane * kRecPad + 7] = 0.0;
+    // LDS operations of one wave complete in order; the fences keep the
+    // compiler from moving the accesses across (records written above are read
+    // by other lanes below, and rewritten in the next chunk)
+    wave_lds_sync();
+    double* o = out8 + 8 * wb;
+    double t8[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = q * 64 + lane;  // element of the wave's 512 doubles
+      t8[q] = st[(e >> 3) * kRecPad + (e & 7)];
+    }
+    if (wb + 64 <= n) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) o[q * 64 + lane] = t8[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (wb + ((q * 64 + lane) >> 3) < n) o[q * 64 + lane] = t8[q];
+    }
+    wave_lds_sync();
   }
   if (mine) atomicAdd(&cnt, mine);
   __syncthreads();

@@ -201,6 +201,20 @@ __global__ void k_convert(const cpx<double>* __restrict__ in, cpx<float>* __rest
 
 }  // namespace
 
+// Axes along which the solve-path primitive of an E component is mirrored
+// (k_prim_offset): mev, where xlafun2 is even (its first argument), and mod,
+// where it is odd (its third argument); -1 where the axis has an offset or odd
+// length, and for phi or a Green function returned to the caller.
+static void prim_mirrors(const GreenSpec& s, int& mev, int& mod) {
+  auto mirrorable = [&](int ax) { return s.offset[ax] == 0.0 && s.L[ax] % 2 == 0; };
+  mev = mod = -1;
+  if (s.zero_raw && s.icomp >= 1) {
+    const int ea = s.icomp - 1, oa = (s.icomp + 1) % 3;
+    if (mirrorable(ea)) mev = ea;
+    if (mirrorable(oa)) mod = oa;
+  }
+}
+
 // primitive F of an offset-kind Green function (open_spacecharge_mod.f90:662-691)
 static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F) {
   const double dx = s.delta[0], dy = s.delta[1], dz = s.delta[2] * s.gamma;
@@ -217,13 +231,8 @@ static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F) {
   // mirror along their own axis, odd mirror along the axis of xlafun2's third
   // argument, see k_prim_offset); the returned-to-caller Green function
   // (zero_raw == 0) keeps upstream's coordinate arithmetic
-  auto mirrorable = [&](int ax) { return s.offset[ax] == 0.0 && s.L[ax] % 2 == 0; };
-  int mev = -1, mod = -1;
-  if (s.zero_raw && s.icomp >= 1) {
-    const int ea = s.icomp - 1, oa = (s.icomp + 1) % 3;  // first / third argument of xlafun2
-    if (mirrorable(ea)) mev = ea;
-    if (mirrorable(oa)) mod = oa;
-  }
+  int mev, mod;
+  prim_mirrors(s, mev, mod);
   dim3 gp = g3;
   if (mev == 0 || mod == 0) gp.x = unsigned((s.L[0] / 2 + 63) / 64);
   if (mev == 1 || mod == 1) gp.y = unsigned(s.L[1] / 2);
@@ -264,18 +273,25 @@ void build_green_real(osc_ctx* c, const GreenSpec& s, double* g) {
   }
 }
 
+// With a corner difference and sym (per axis: 0, +1 even, -1 odd about
+// L/2 - 1, last plane zero), lines along y and z are computed only for the
+// lower half of a symmetric axis until that axis is transformed, whose pass
+// rebuilds the upper half as it loads (AxisIO::mh).
 template <typename T>
 void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const SpecDims& d,
-                           const CornerDiff* cd) {
+                           const CornerDiff* cd, const int* sym) {
   const int L0 = d.L[0], L1 = d.L[1], L2 = d.L[2];
   ProfScope ps(c, ST_GREEN_FFT);
+  const bool hy = cd && sym && sym[1] != 0 && L1 % 2 == 0;
+  const bool hz = cd && sym && sym[2] != 0 && L2 % 2 == 0 && cd->sh == 0;
+  const int Yh = hy ? L1 / 2 : L1, Zh = hz ? L2 / 2 : L2;
   R2CIO r{};
   r.in_sy = L0;
   r.in_sz = int64_t(L0) * L1;
   r.out_sy = d.KXP;
   r.out_sz = int64_t(d.KXP) * L1;
-  r.nrows = int64_t(L1) * L2;
-  r.ny_rows = L1;
+  r.nrows = int64_t(Yh) * Zh;
+  r.ny_rows = Yh;
   r.n_in = L0;
   if (cd)
     launch_xr2c_diff<T>(L0, g, spec, r, *cd, twiddles<T>(c, L0), c->stream);
@@ -289,7 +305,9 @@ void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const Spec
   ay.n_in = L1;
   ay.w0 = 0;
   ay.n_out = L1;
-  launch_axis<T>(L1, -1, spec, spec, ay, L2, 1, twiddles<T>(c, L1), c->stream);
+  ay.mh = hy ? L1 / 2 : 0;
+  ay.ms = hy ? sym[1] : 0;
+  launch_axis<T>(L1, -1, spec, spec, ay, Zh, 1, twiddles<T>(c, L1), c->stream);
   AxisIO az{};
   az.s_b = az.d_b = 1;
   az.s_p = az.d_p = int64_t(d.KXP) * L1;
@@ -298,12 +316,14 @@ void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const Spec
   az.n_in = L2;
   az.w0 = 0;
   az.n_out = L2;
+  az.mh = hz ? L2 / 2 : 0;
+  az.ms = hz ? sym[2] : 0;
   launch_axis<T>(L2, -1, spec, spec, az, L1, 1, twiddles<T>(c, L2), c->stream);
 }
 template void spectrum_forward_full<double>(osc_ctx*, const double*, cpx<double>*, const SpecDims&,
-                                            const CornerDiff*);
+                                            const CornerDiff*, const int*);
 template void spectrum_forward_full<float>(osc_ctx*, const double*, cpx<float>*, const SpecDims&,
-                                           const CornerDiff*);
+                                           const CornerDiff*, const int*);
 
 void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_t n) {
   ProfScope ps(c, ST_GREEN_FFT);
@@ -344,8 +364,15 @@ const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) 
       ProfScope ps(c, ST_GREEN_EVAL);
       build_offset_primitive(c, s, g.as<double>());
     }
-    const CornerDiff cd{s.L[2], s.out_shift_z, s.zero_raw, 0};
-    spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d, &cd);
+    // the corner difference of a primitive that is even along an axis is odd
+    // about L/2 - 1 there, and even where the primitive is odd
+    int mev, mod;
+    prim_mirrors(s, mev, mod);
+    int sym[3] = {0, 0, 0};
+    if (mev >= 0) sym[mev] = -1;
+    if (mod >= 0) sym[mod] = 1;
+    const CornerDiff cd{s.L[1], s.L[2], s.out_shift_z, s.zero_raw};
+    spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d, &cd, sym);
   } else {
     build_green_real(c, s, g.as<double>());
     spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d);

@@ -207,10 +207,11 @@ void build_green_real(osc_ctx* c, const GreenSpec& s, double* g);  // (L0,L1,L2)
 template <typename T>
 const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive);
 // forward R2C of a real (L0,L1,L2) FP64 array into a half spectrum of precision T
-// (with cd: g is a primitive and the x pass takes its corner difference)
+// (with cd: g is a primitive and the x pass takes its corner difference;
+// sym: per-axis mirror symmetry of that difference, 0 / +1 even / -1 odd)
 template <typename T>
 void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const SpecDims& d,
-                           const CornerDiff* cd = nullptr);
+                           const CornerDiff* cd = nullptr, const int* sym = nullptr);
 void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_t n);
 // download a half spectrum and expand it to the upstream full-array
 // convention F_s[g] (sign s[d] on axis d)
