@@ -83,6 +83,10 @@ def stage_bytes(stage, s, n3, L, P, nout, nspec):
         return nout * (c * KX * L[1] * nz + c * KX * ny * nz)
     if stage == "fft.inv_x":
         return nout * c * KX * ny * nz + 8 * n * 5
+    if stage == "deposit.cells":
+        # B_deposit: the particle coordinates once, the charge grid once; the
+        # kernel of this stage is the one that reads the particles in cell order
+        return 24 * P + s * n
     if stage == "gather":
         # compulsory part of the field gather (the unpermute pass is separate)
         return 24 * P + 48 * P + 48 * n
@@ -391,7 +395,8 @@ def cpu_child(args):
 
 # stage of the per-stage HIP-event timing -> the one kernel it times
 STAGE_KERNEL = {"gather": "k_interp_cells", "gather.out": "k_unpermute", "fft.inv_zmul": "k_zinv<double",
-                "fft.inv_x": "k_xc2r<double", "fft.fwd_x": "k_xr2c<double"}
+                "fft.inv_x": "k_xc2r<double", "fft.fwd_x": "k_xr2c<double",
+                "deposit.cells": "k_seg_reduce"}
 
 
 def traffic_child(args):

@@ -89,7 +89,7 @@ int osc_abi_version(void);
  * bracketed by events of its stage; osc_ctx_profile_read synchronises, adds
  * the elapsed times (ms) and launch counts of each stage into the arrays
  * (OSC_NSTAGE entries) and clears the records. */
-#define OSC_NSTAGE 19
+#define OSC_NSTAGE 20
 int osc_ctx_set_profiling(osc_ctx* ctx, int on);
 int osc_ctx_profile_read(osc_ctx* ctx, double* ms, int64_t* launches);
 const char* osc_stage_name(int stage);

@@ -226,7 +226,7 @@ class Context:
     def synchronize(self):
         self.check(self.lib.osc_ctx_synchronize(self.h))
 
-    NSTAGE = 19
+    NSTAGE = 20
 
     def set_profiling(self, on: bool):
         self.check(self.lib.osc_ctx_set_profiling(self.h, int(bool(on))))

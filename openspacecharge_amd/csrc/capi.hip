@@ -171,7 +171,7 @@ OSC_API const char* osc_stage_name(int s) {
       "deposit.bbox", "deposit.keys", "deposit.sort", "deposit.cells", "deposit.nodes",
       "green.eval",   "green.fft",    "fft.fwd_x",    "fft.fwd_y",     "fft.fwd_z",
       "fft.inv_zmul", "fft.inv_y",    "fft.inv_x",    "gather",        "fd_bfield",
-      "other",        "gather.pack",  "gather.sort",  "gather.out"};
+      "other",        "gather.pack",  "gather.sort",  "gather.out",    "deposit.fix"};
   return (s >= 0 && s < ST_COUNT) ? names[s] : "?";
 }
 

@@ -269,9 +269,20 @@ __device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t* s) {
   return v;
 }
 
-// One pass: each block ranks its tile of RS_TILE pairs by digit (stable: lane
-// order within a wave, wave order, then iteration order), stages the
-// tile in LDS in that order, and writes it out so that consecutive threads
+// the LDS operations of one wave run in program order; the fences keep the
+// compiler from moving accesses across (other lanes read what was written)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
+// One pass over a tile of RS_TILE pairs per block.  Wave w ranks its own
+// contiguous share of the tile (RS_TILE / RS_W pairs, 64 per step, held in
+// registers) with wave-private digit counters: the rank of a pair is the
+// count of earlier pairs of the share with its digit (earlier steps, then
+// lower lanes).  The tile's pairs are then placed in LDS in stable digit order
+// (digit, then wave, then rank) and written out so that consecutive threads
 // store consecutive slots of one digit's run.  col[d * nb + b] is the
 // exclusive scan over blocks of digit d's per-block counts (k_digit_scan) and
 // total[d] that digit's count; the slots of digit d start after all smaller
@@ -281,33 +292,21 @@ __global__ void __launch_bounds__(RS_T)
                  uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n, int shift,
                  int bits, const uint32_t* __restrict__ col, const uint32_t* __restrict__ total,
                  int nb) {
+  constexpr int SHARE = RS_TILE / RS_W;  // pairs per wave
   __shared__ uint32_t lk[RS_TILE], lv[RS_TILE];
-  __shared__ uint32_t run[256], gofs[256], scan[256];
-  __shared__ uint32_t wc[RS_W][256];
+  __shared__ uint32_t wcnt[RS_W][256];  // wave counters, then each wave's first local slot
+  __shared__ uint32_t gofs[256], scan[RS_W];
   const uint32_t mask = (1u << bits) - 1u;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
-  {
-    const int d = threadIdx.x;  // RS_T == 256 digits at most
-    const uint32_t tot = d <= int(mask) ? total[d] : 0u;
-    const uint32_t dbase = block_scan_incl(tot, scan) - tot;  // first slot of digit d
-    // this block's count of each digit, and their exclusive scan over digits
-    uint32_t cnt = 0, g = 0;
-    if (d <= int(mask)) {
-      const int64_t f = int64_t(d) * nb + blockIdx.x;
-      const uint32_t c0 = col[f];
-      cnt = (int(blockIdx.x) + 1 < nb ? col[f + 1] : tot) - c0;
-      g = dbase + c0;
-    }
-    const uint32_t start = block_scan_incl(cnt, scan) - cnt;  // exclusive
-    run[d] = start;
-    gofs[d] = g - start;  // global slot of local slot j of digit d: gofs[d] + j
-  }
-  const int64_t base = int64_t(blockIdx.x) * RS_TILE;
-  for (int it = 0; it < RS_I; ++it) {
-    for (int j = threadIdx.x; j < RS_W * 256; j += RS_T) (&wc[0][0])[j] = 0u;
-    __syncthreads();
-    const int64_t i = base + it * RS_T + threadIdx.x;
+  for (int j = threadIdx.x; j < RS_W * 256; j += RS_T) (&wcnt[0][0])[j] = 0u;
+  __syncthreads();
+  const int64_t wbase = int64_t(blockIdx.x) * RS_TILE + int64_t(wave) * SHARE;
+  uint32_t* cnt = wcnt[wave];
+  uint32_t key[SHARE / 64], val[SHARE / 64], rk[SHARE / 64];
+#pragma unroll
+  for (int it = 0; it < SHARE / 64; ++it) {
+    const int64_t i = wbase + it * 64 + lane;
     const bool valid = i < n;
     const uint32_t k = valid ? kin[i] : 0u;
     const uint32_t v = valid ? (vin ? vin[i] : uint32_t(i)) : 0u;
@@ -318,26 +317,47 @@ __global__ void __launch_bounds__(RS_T)
       const uint64_t bal = __ballot(bit);
       peers &= bit ? bal : ~bal;
     }
-    const uint32_t rank = __popcll(peers & lt);
-    if (valid && rank == 0) wc[wave][d] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = run[d] + rank;
-      for (int w = 0; w < wave; ++w) pos += wc[w][d];
-      lk[pos] = k;
-      lv[pos] = v;
-    }
-    __syncthreads();
-    for (int dd = threadIdx.x; dd <= int(mask); dd += RS_T) {
-      uint32_t t = 0;
-      for (int w = 0; w < RS_W; ++w) t += wc[w][dd];
-      run[dd] += t;
-    }
-    __syncthreads();
+    const uint32_t r = __popcll(peers & lt);
+    const uint32_t before = cnt[d];
+    wave_lds_sync();
+    if (valid && r == 0) cnt[d] = before + uint32_t(__popcll(peers));
+    wave_lds_sync();
+    key[it] = k;
+    val[it] = v;
+    rk[it] = before + r;
   }
-  const int64_t rem = n - base;
-  const int cnt = rem < RS_TILE ? int(rem) : RS_TILE;
-  for (int j = threadIdx.x; j < cnt; j += RS_T) {
+  __syncthreads();
+  {
+    const int d = threadIdx.x;  // RS_T == 256 digits at most
+    uint32_t mine = 0;  // this block's count of digit d; wave prefixes in place
+#pragma unroll
+    for (int w = 0; w < RS_W; ++w) {
+      const uint32_t c = wcnt[w][d];
+      wcnt[w][d] = mine;
+      mine += c;
+    }
+    const uint32_t tot = d <= int(mask) ? total[d] : 0u;
+    const uint32_t dbase = block_scan_incl(tot, scan) - tot;  // first slot of digit d
+    const uint32_t start = block_scan_incl(mine, scan) - mine;  // first local slot of digit d
+#pragma unroll
+    for (int w = 0; w < RS_W; ++w) wcnt[w][d] += start;
+    const uint32_t g = d <= int(mask) ? dbase + col[int64_t(d) * nb + blockIdx.x] : 0u;
+    gofs[d] = g - start;  // global slot of local slot j of digit d: gofs[d] + j
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < SHARE / 64; ++it) {
+    const int64_t i = wbase + it * 64 + lane;
+    if (i < n) {
+      const uint32_t pos = wcnt[wave][(key[it] >> shift) & mask] + rk[it];
+      lk[pos] = key[it];
+      lv[pos] = val[it];
+    }
+  }
+  __syncthreads();
+  const int64_t rem = n - int64_t(blockIdx.x) * RS_TILE;
+  const int cnt_t = rem < RS_TILE ? int(rem) : RS_TILE;
+  for (int j = threadIdx.x; j < cnt_t; j += RS_T) {
     const uint32_t k = lk[j];
     const uint32_t g = gofs[(k >> shift) & mask] + uint32_t(j);
     kout[g] = k;
@@ -793,7 +813,6 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
     ProfScope ps(c, ST_DEP_SORT);
     radix_sort(c, ka, va, kb, vb, n, sp, hist, SortPayload{}, in_b);
   }
-  ProfScope ps_cells(c, ST_DEP_CELLS);
   const uint32_t* sk = in_b ? kb.as<uint32_t>() : ka.as<uint32_t>();
   const uint32_t* sv = in_b ? vb.as<uint32_t>() : va.as<uint32_t>();
   // corner sums of every cell (zero where no particle), wave partials of runs
@@ -801,13 +820,19 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   const int64_t nw = (n + 63) / 64 + 1;
   DevBuf S(c->pool, sizeof(double) * 8 * ncell), head(c->pool, sizeof(double) * 8 * nw),
       tail(c->pool, sizeof(double) * 8 * nw), heavy(c->pool, sizeof(uint32_t) * nw);
-  fill_f64(c, S.as<double>(), 8 * ncell, 0.0);
+  {
+    ProfScope pf(c, ST_DEP_FIX);
+    fill_f64(c, S.as<double>(), 8 * ncell, 0.0);
+  }
   if (n > 0) {
+    ProfScope ps_cells(c, ST_DEP_CELLS);
     hipLaunchKernelGGL(k_seg_reduce, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk, sv,
                        rec.as<const double4>(), keep ? srec.as<double4>() : nullptr,
                        keep ? inv.as<uint32_t>() : nullptr, n, cnt, cp, S.as<double>(),
                        head.as<double>(), tail.as<double>());
     OSC_HIP(hipGetLastError());
+    ps_cells.stop();
+    ProfScope pf(c, ST_DEP_FIX);
     hipLaunchKernelGGL(k_seg_fix, dim3(nblk(nw)), dim3(kThreads), 0, c->stream, sk, n, cnt, ncell,
                        head.as<const double>(), tail.as<const double>(), S.as<double>(),
                        heavy.as<uint32_t>(), cnt + 1);
@@ -817,7 +842,6 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
                        heavy.as<const uint32_t>(), cnt + 1);
     OSC_HIP(hipGetLastError());
   }
-  ps_cells.stop();
   {
     ProfScope ps(c, ST_DEP_NODES);
     hipLaunchKernelGGL(k_node, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,

@@ -93,6 +93,7 @@ enum Stage : int {
   ST_GATHER_PACK,
   ST_GATHER_SORT,
   ST_GATHER_OUT,
+  ST_DEP_FIX,
   ST_COUNT
 };
 
