@@ -9,6 +9,11 @@ interpolate_field of E and B at all 10M particles.  Particles, meshes and
 fields are resident in HBM before the timed region; the Green function is
 rebuilt in every step, as the upstream solver does (no cached spectra).
 
+Each precision runs the step K times back to back between two device
+synchronisations (the solves/s figures), then K times each waited for (p50,
+p99 latency), then K times with per-stage HIP-event timing (stage times and
+the roofline kernel's launch duration).
+
 Prints ONE JSON line.  `value` is FP64 drop-in solves/s; the same line
 carries the FP32-convolution rate, its field error against the FP64 result,
 the per-stage times, the roofline of the dominant kernel and the CPU baseline
@@ -134,18 +139,26 @@ def gpu_leg(args, x, y, z):
         t31 = time.perf_counter()
         for _ in range(args.warmup):
             step()
-        torch.cuda.synchronize()
-        ctx.profile_read()
-        ctx.set_profiling(True)
+        # throughput (the headline): K steps back to back
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        # latency: each step waited for
         lat = []
         for _ in range(args.steps):
             ts = time.perf_counter()
             step()
             torch.cuda.synchronize()
             lat.append(time.perf_counter() - ts)
-        t1 = time.perf_counter()
+        # per-stage HIP-event timing (events recorded on the context's stream)
+        ctx.profile_read()
+        ctx.set_profiling(True)
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
         prof = ctx.profile_read()
         ctx.set_profiling(False)
         # cached-Green variant (repeated mesh geometry), untimed by the headline

@@ -47,7 +47,10 @@ void zinv_t(const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, int Ly, int nout
             hipStream_t s) {
   constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
   dim3 g = grid3(cdiv(a.nb, BX), Ly, nout);
-  hipLaunchKernelGGL((k_zinv<T, L>), g, dim3(NT), 0, s, rho, out, a, tw);
+  if (a.zreal)
+    hipLaunchKernelGGL((k_zinv_g<T, L>), g, dim3(NT), 0, s, rho, out, a, tw);
+  else
+    hipLaunchKernelGGL((k_zinv<T, L>), g, dim3(NT), 0, s, rho, out, a, tw);
 }
 }  // namespace
 
@@ -144,6 +147,7 @@ void launch_zinv<T>(int L, const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, i
     OSC_FOR_EACH_L(OSC_CASE)
 #undef OSC_CASE
     default: {
+      OSC_REQUIRE(!a.zreal, OSC_ERR_FFT_LENGTH, "z-real Green arrays need a planned length");
       GenericPlan pl = generic_plan(L);
       dim3 g = grid3(cdiv(a.nb, kGenBX), Ly, nout);
       hipLaunchKernelGGL((k_zinv_gen<T>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, rho, out,

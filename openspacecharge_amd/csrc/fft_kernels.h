@@ -78,6 +78,9 @@ struct Term {
   const cpx<T>* g;
   T sgn;
   int mir;  // bit 0: x, bit 1: y, bit 2: z
+  // z-real Green lines (ZinvIO::zreal): zmh > 0 when only z < zmh is stored,
+  // the line being mirror-symmetric about Lz/2 - 1 with sign zms
+  int zmh, zms;
 };
 template <typename T>
 struct OutTerms {
@@ -89,6 +92,10 @@ struct ZinvIO {
   int64_t sy, sz;         // spectrum strides (KXP, KXP*Ly)
   int64_t o_sy, o_sz, o_so;  // output strides
   int Ly, Lz, nb, w0, n_out, nout;
+  // 1: every output has one term, whose Green array is transformed along x
+  // and y only; its z transform is taken per line inside the multiply
+  // (k_zinv_g)
+  int zreal;
   OutTerms<T> out[kMaxOut];
 };
 
@@ -418,6 +425,85 @@ __global__ void __launch_bounds__((Geo<T, L>::NT))
   f.run(lds, tw);
   cpx<T>* ob = out + o * a.o_so + kx + ky * a.o_sy;
   f.store([&](int, int p, cpx<T> v) {
+    const int q = p - a.w0;
+    if (ok && q >= 0 && q < a.n_out) ob[q * a.o_sz] = v;
+  });
+}
+
+// element kz of the rho_hat factor of a term (the mirror and conjugation
+// rules of combine)
+template <typename T>
+__device__ __forceinline__ cpx<T> rho_term(const cpx<T>* __restrict__ rho, const ZinvIO<T>& a,
+                                           const Term<T>& tm, int kx, int ky, int kz) {
+  bool my, mz, cj;
+  if (tm.mir & 1) {
+    my = !(tm.mir & 2);
+    mz = !(tm.mir & 4);
+    cj = true;
+  } else {
+    my = tm.mir & 2;
+    mz = tm.mir & 4;
+    cj = false;
+  }
+  const int yy = my ? (ky == 0 ? 0 : a.Ly - ky) : ky;
+  const int zz = mz ? (kz == 0 ? 0 : a.Lz - kz) : kz;
+  cpx<T> r = rho[kx + yy * a.sy + zz * a.sz];
+  if (cj) r.y = -r.y;
+  return r;
+}
+
+// element p of a z-real Green line (Term::zmh mirror rule)
+template <typename T, int L>
+__device__ __forceinline__ cpx<T> zreal_in(const cpx<T>* gl, int64_t sz, const Term<T>& tm, int p) {
+  if (tm.zmh > 0 && p >= tm.zmh) {
+    if (p == L - 1) return mk<T>(0, 0);
+    const cpx<T> v = gl[(L - 2 - p) * sz];
+    return tm.zms < 0 ? mk<T>(-v.x, -v.y) : v;
+  }
+  return gl[p * sz];
+}
+
+// The multiply and inverse z pass for outputs of one term whose Green array
+// is z-real: per (kx, ky) line the Green line is transformed along z here (it
+// is never stored as a full spectrum), multiplied by its rho_hat factor into
+// LDS and transformed back.
+template <typename T, int L>
+__global__ void __launch_bounds__((Geo<T, L>::NT))
+    k_zinv_g(const cpx<T>* __restrict__ rho, cpx<T>* __restrict__ out, ZinvIO<T> a,
+             const cpx<T>* __restrict__ tw) {
+  constexpr int BX = Geo<T, L>::BX;
+  using FF = FftTile<T, L, -1, BX, false>;
+  using FI = FftTile<T, L, 1, BX, false>;
+  static_assert(FF::LDS == FI::LDS, "forward and inverse tiles differ");
+  __shared__ cpx<T> lds[FF::LDS];
+  const int o = blockIdx.z, ky = blockIdx.y;
+  const Term<T> tm = a.out[o].t[0];
+  {
+    FF ff;
+    const int kx = blockIdx.x * BX + ff.b;
+    const bool ok = kx < a.nb;
+    const cpx<T>* gl = tm.g + (ok ? kx + int64_t(ky) * a.sy : 0);
+    ff.load([&](int, int p) { return ok ? zreal_in<T, L>(gl, a.sz, tm, p) : mk<T>(0, 0); });
+    ff.run(lds, tw);
+    __syncthreads();  // the transform's last reads of lds
+    ff.store([&](int b, int p, cpx<T> g) {
+      cpx<T> q = mk<T>(0, 0);
+      if (ok) {
+        q = mul(rho_term(rho, a, tm, kx, ky, p), g);
+        q = mk(tm.sgn * q.x, tm.sgn * q.y);
+      }
+      lds[FF::lidx(b, p)] = q;
+    });
+  }
+  __syncthreads();
+  FI fi;
+  const int kx = blockIdx.x * BX + fi.b;
+  const bool ok = kx < a.nb;
+  fi.load([&](int b, int p) { return lds[FI::lidx(b, p)]; });
+  __syncthreads();
+  fi.run(lds, tw);
+  cpx<T>* ob = out + o * a.o_so + kx + ky * a.o_sy;
+  fi.store([&](int, int p, cpx<T> v) {
     const int q = p - a.w0;
     if (ok && q >= 0 && q < a.n_out) ob[q * a.o_sz] = v;
   });

@@ -279,7 +279,7 @@ void build_green_real(osc_ctx* c, const GreenSpec& s, double* g) {
 // rebuilds the upper half as it loads (AxisIO::mh).
 template <typename T>
 void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const SpecDims& d,
-                           const CornerDiff* cd, const int* sym) {
+                           const CornerDiff* cd, const int* sym, GreenRef* zr) {
   const int L0 = d.L[0], L1 = d.L[1], L2 = d.L[2];
   ProfScope ps(c, ST_GREEN_FFT);
   const bool hy = cd && sym && sym[1] != 0 && L1 % 2 == 0;
@@ -308,6 +308,13 @@ void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const Spec
   ay.mh = hy ? L1 / 2 : 0;
   ay.ms = hy ? sym[1] : 0;
   launch_axis<T>(L1, -1, spec, spec, ay, Zh, 1, twiddles<T>(c, L1), c->stream);
+  if (zr) {
+    zr->g = spec;
+    zr->zr = 1;
+    zr->zmh = hz ? L2 / 2 : 0;
+    zr->zms = hz ? sym[2] : 0;
+    return;
+  }
   AxisIO az{};
   az.s_b = az.d_b = 1;
   az.s_p = az.d_p = int64_t(d.KXP) * L1;
@@ -321,9 +328,9 @@ void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const Spec
   launch_axis<T>(L2, -1, spec, spec, az, L1, 1, twiddles<T>(c, L2), c->stream);
 }
 template void spectrum_forward_full<double>(osc_ctx*, const double*, cpx<double>*, const SpecDims&,
-                                            const CornerDiff*, const int*);
+                                            const CornerDiff*, const int*, GreenRef*);
 template void spectrum_forward_full<float>(osc_ctx*, const double*, cpx<float>*, const SpecDims&,
-                                           const CornerDiff*, const int*);
+                                           const CornerDiff*, const int*, GreenRef*);
 
 void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_t n) {
   ProfScope ps(c, ST_GREEN_FFT);
@@ -333,10 +340,12 @@ void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_
 
 // Spectrum of the Green function of s in the context's precision: the real
 // Green function is always built in FP64 (IGF primitives and 8-corner
-// differences); an FP32 context transforms it in FP32.  Cached per
-// precision under the full specification.
+// differences); an FP32 context transforms it in FP32.  When the z length has
+// a compile-time plan the z transform is left to conv_solve, which takes it
+// line by line inside the multiply (the array is z-real, GreenRef::zr).
+// Cached per precision under the full specification.
 template <typename T>
-const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) {
+GreenRef green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) {
   const SpecDims d = SpecDims::of(s.L);
   const int64_t N = int64_t(s.L[0]) * s.L[1] * s.L[2];
   std::string key(reinterpret_cast<const char*>(&s), sizeof(s));
@@ -350,13 +359,15 @@ const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) 
       DevBuf& have = sizeof(T) == 4 ? hit->f32 : hit->f64;
       if (have.p) {
         ++gc.hits;
-        return have.as<cpx<T>>();
+        return GreenRef{have.p, hit->zr, hit->zmh, hit->zms};
       }
     }
     ++gc.misses;
   }
   DevBuf g(c->pool, sizeof(double) * N);
   DevBuf sp(c->pool, sizeof(cpx<T>) * d.size());
+  GreenRef ref{};
+  GreenRef* zr = fft_has_plan(s.L[2]) ? &ref : nullptr;
   if (s.kind == GK_OFFSET && fft_has_plan(s.L[0])) {
     // the x pass takes the 8-corner difference of the primitive as it loads
     // its rows, so the difference array is never stored
@@ -372,15 +383,16 @@ const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) 
     if (mev >= 0) sym[mev] = -1;
     if (mod >= 0) sym[mod] = 1;
     const CornerDiff cd{s.L[1], s.L[2], s.out_shift_z, s.zero_raw};
-    spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d, &cd, sym);
+    spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d, &cd, sym, zr);
   } else {
     build_green_real(c, s, g.as<double>());
-    spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d);
+    spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d, nullptr, nullptr, zr);
   }
   g.reset();
+  ref.g = sp.p;
   if (!gc.enable) {
     keepalive = std::move(sp);
-    return keepalive.as<cpx<T>>();
+    return ref;
   }
   GreenCache::Entry* ep = hit;
   if (!ep) {
@@ -388,6 +400,9 @@ const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) 
     ep = e.get();
     gc.map[key] = std::move(e);
   }
+  ep->zr = ref.zr;
+  ep->zmh = ref.zmh;
+  ep->zms = ref.zms;
   const size_t bytes = sizeof(cpx<T>) * d.size();
   (sizeof(T) == 4 ? ep->f32 : ep->f64) = std::move(sp);
   ep->bytes += bytes;
@@ -399,10 +414,10 @@ const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) 
     gc.evict_to(gc.max_bytes);
     ep->last = gc.tick;
   }
-  return (sizeof(T) == 4 ? ep->f32 : ep->f64).template as<cpx<T>>();
+  return ref;
 }
-template const cpx<double>* green_spectrum<double>(osc_ctx*, const GreenSpec&, DevBuf&);
-template const cpx<float>* green_spectrum<float>(osc_ctx*, const GreenSpec&, DevBuf&);
+template GreenRef green_spectrum<double>(osc_ctx*, const GreenSpec&, DevBuf&);
+template GreenRef green_spectrum<float>(osc_ctx*, const GreenSpec&, DevBuf&);
 
 // full upstream array F_s[g](k) = F_-[g](sigma_s k), sigma flipping axes with s=+1
 void half_to_upstream(osc_ctx* c, const cpx<double>* spec_dev, const SpecDims& d, const int s[3],

@@ -44,10 +44,20 @@ struct GreenSpec {
 };
 GreenSpec make_green_spec();
 
+// A Green function as conv_solve consumes it: a half spectrum, or (zr) an
+// array transformed along x and y only, whose z transform conv_solve takes
+// per line; then zmh > 0 when only z < zmh is stored, the lines being
+// mirror-symmetric about L/2 - 1 with sign zms.
+struct GreenRef {
+  const void* g = nullptr;
+  int zr = 0, zmh = 0, zms = 0;
+};
+
 class GreenCache {
  public:
   struct Entry {
     DevBuf f64, f32;
+    int zr = 0, zmh = 0, zms = 0;
     size_t bytes = 0;
     uint64_t last = 0;
   };
@@ -204,15 +214,18 @@ bool is_device_ptr(const void* p);
 
 // ---- green.hip --------------------------------------------------------------
 void build_green_real(osc_ctx* c, const GreenSpec& s, double* g);  // (L0,L1,L2) real
-// half spectrum of a real-space Green array in precision T; cached by spec
+// spectrum of a Green function in precision T (z-real when the z length has a
+// compile-time plan); cached by spec
 template <typename T>
-const cpx<T>* green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive);
+GreenRef green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive);
 // forward R2C of a real (L0,L1,L2) FP64 array into a half spectrum of precision T
 // (with cd: g is a primitive and the x pass takes its corner difference;
-// sym: per-axis mirror symmetry of that difference, 0 / +1 even / -1 odd)
+// sym: per-axis mirror symmetry of that difference, 0 / +1 even / -1 odd;
+// with zr: the z pass is left out and *zr describes the stored z lines)
 template <typename T>
 void spectrum_forward_full(osc_ctx* c, const double* g, cpx<T>* spec, const SpecDims& d,
-                           const CornerDiff* cd = nullptr, const int* sym = nullptr);
+                           const CornerDiff* cd = nullptr, const int* sym = nullptr,
+                           GreenRef* zr = nullptr);
 void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_t n);
 // download a half spectrum and expand it to the upstream full-array
 // convention F_s[g] (sign s[d] on axis d)
@@ -225,7 +238,7 @@ void upstream_to_half(osc_ctx* c, const double* full_host, const SpecDims& d, co
 struct OutSpec {
   int nt = 0;
   struct {
-    const void* g;
+    GreenRef g;
     double sgn;
     int mir;
   } t[kMaxTerm];

@@ -174,13 +174,51 @@ void conv_solve(osc_ctx* c, const double* rho, const int n[3], const SpecDims& d
   zi.w0 = win[2];
   zi.n_out = nz;
   zi.nout = nout;
+  zi.zreal = outs[0].t[0].g.zr;
+  bool multi = false;
   for (int o = 0; o < nout; ++o) {
     zi.out[o].nt = outs[o].nt;
+    multi = multi || outs[o].nt > 1;
     for (int t = 0; t < outs[o].nt; ++t) {
-      zi.out[o].t[t].g = static_cast<const cpx<T>*>(outs[o].t[t].g);
+      const GreenRef& gr = outs[o].t[t].g;
+      OSC_REQUIRE(gr.zr == zi.zreal, OSC_ERR_ARG, "Green arrays of one solve differ in z form");
+      zi.out[o].t[t].g = static_cast<const cpx<T>*>(gr.g);
       zi.out[o].t[t].sgn = T(outs[o].t[t].sgn);
       zi.out[o].t[t].mir = outs[o].t[t].mir;
+      zi.out[o].t[t].zmh = gr.zmh;
+      zi.out[o].t[t].zms = gr.zms;
     }
+  }
+  // outputs summing several terms take full spectra: the z pass of each
+  // distinct z-real Green array into a scratch spectrum
+  std::vector<DevBuf> zfull;
+  if (zi.zreal && multi) {
+    std::map<const void*, const cpx<T>*> done;
+    ProfScope ps(c, ST_GREEN_FFT);
+    for (int o = 0; o < nout; ++o)
+      for (int t = 0; t < zi.out[o].nt; ++t) {
+        Term<T>& tm = zi.out[o].t[t];
+        auto it = done.find(tm.g);
+        if (it == done.end()) {
+          zfull.emplace_back(c->pool, sizeof(cpx<T>) * d.size());
+          cpx<T>* full = zfull.back().template as<cpx<T>>();
+          AxisIO az{};
+          az.s_b = az.d_b = 1;
+          az.s_p = az.d_p = KXP * L1;
+          az.s_o1 = az.d_o1 = KXP;
+          az.nb = d.KX;
+          az.n_in = L2;
+          az.w0 = 0;
+          az.n_out = L2;
+          az.mh = tm.zmh;
+          az.ms = tm.zms;
+          launch_axis<T>(L2, -1, tm.g, full, az, L1, 1, twiddles<T>(c, L2), s);
+          it = done.emplace(tm.g, full).first;
+        }
+        tm.g = it->second;
+        tm.zmh = tm.zms = 0;
+      }
+    zi.zreal = 0;
   }
   {
     ProfScope ps(c, ST_INV_ZMUL);

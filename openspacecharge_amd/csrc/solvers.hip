@@ -12,7 +12,7 @@ constexpr double kFpeiPipe = 8987551744.0;                        // core:783
 constexpr double kClightSp = 299792448.0;                         // open_spacecharge_mod.f90:436
 constexpr double kClightDp = 299792458.0;                         // core:51
 
-OutSpec out1(const void* g, double s1, double* dst1) {
+OutSpec out1(GreenRef g, double s1, double* dst1) {
   OutSpec o;
   o.nt = 1;
   o.t[0] = {g, 1.0, 0};
@@ -23,7 +23,7 @@ OutSpec out1(const void* g, double s1, double* dst1) {
   o.ep.beta = 1.0;
   return o;
 }
-OutSpec out2(const void* g, const void* h, double sh, int mir, double s1, double* dst1) {
+OutSpec out2(GreenRef g, GreenRef h, double sh, int mir, double s1, double* dst1) {
   OutSpec o = out1(g, s1, dst1);
   o.nt = 2;
   o.t[1] = {h, sh, mir};
@@ -76,7 +76,7 @@ template <typename T>
 struct Specs {
   std::vector<DevBuf> keep;
   explicit Specs(size_t n) : keep(n) {}
-  const void* get(osc_ctx* c, const GreenSpec& s, size_t slot) {
+  GreenRef get(osc_ctx* c, const GreenSpec& s, size_t slot) {
     return green_spectrum<T>(c, s, keep[slot]);
   }
 };
@@ -129,7 +129,7 @@ void sc3d_t(osc_ctx* c, const osc_mesh& m, const double* rho, const double* offs
     bc = beta / kClightSp;
   }
   Specs<T> sp(6);
-  const void* G[3];
+  GreenRef G[3];
   for (int ic = 1; ic <= 3; ++ic)
     G[ic - 1] = sp.get(c, offset_spec(L, m.delta, m.gamma, ic, off, 0), ic - 1);
   std::vector<OutSpec> outs;
@@ -143,7 +143,7 @@ void sc3d_t(osc_ctx* c, const osc_mesh& m, const double* rho, const double* offs
     // image solve of -rho flipped in z with the offset of :472
     double offi[3] = {off[0], off[1], off[2]};
     offi[2] = offi[2] + 2 * m.min[2] + (m.max[2] - m.min[2]);
-    const void* H[3];
+    GreenRef H[3];
     for (int ic = 1; ic <= 3; ++ic)
       H[ic - 1] = sp.get(c, offset_spec(L, m.delta, m.gamma, ic, offi, n[2] - 1), 3 + ic - 1);
     for (int ic = 0; ic < 3; ++ic) outs.push_back(out2(G[ic], H[ic], -1.0, 4, s1, ef + ic * nt));
@@ -231,8 +231,8 @@ int cathode_t(osc_ctx* c, const osc_mesh& m, double* rho, int direct, int igf, i
   Specs<T> sp(8);
   std::vector<OutSpec> outs;
   if (direct == 0) {
-    const void* g = sp.get(c, legacy_spec(m.nlo, m.nhi, m.npad, m.delta, m.gamma, 0, igf), 0);
-    const void* h = sp.get(c, image_spec(0), 1);
+    const GreenRef g = sp.get(c, legacy_spec(m.nlo, m.nhi, m.npad, m.delta, m.gamma, 0, igf), 0);
+    const GreenRef h = sp.get(c, image_spec(0), 1);
     DevBuf phs(c->pool, sizeof(double) * nt), es(c->pool, sizeof(double) * 3 * nt);
     outs.push_back(out2(g, h, -1.0, 4, s1, phi));                 // phi = phi_free - phi_img
     outs.push_back(out2(g, h, 1.0, 4, s1, phs.as<double>()));     // phi_free + phi_img
@@ -242,8 +242,8 @@ int cathode_t(osc_ctx* c, const osc_mesh& m, double* rho, int direct, int igf, i
     // B from E + 2 E_img = grad(phi_free + phi_img) (core:1076-1077)
     bfield_core(c, es.as<double>(), nt, m.gamma, bf);
   } else {
-    const void* G[3];
-    const void* H[3];
+    GreenRef G[3];
+    GreenRef H[3];
     for (int ic = 1; ic <= 3; ++ic) {
       G[ic - 1] = sp.get(c, legacy_spec(m.nlo, m.nhi, m.npad, m.delta, m.gamma, ic, igf), ic - 1);
       H[ic - 1] = sp.get(c, image_spec(ic), 3 + ic - 1);
@@ -280,7 +280,7 @@ int pipe_t(osc_ctx* c, const osc_mesh& m, int direct, const double* rho, double*
     return OSC_ERR_PIPE_DIRECT;
   }
   const SpecDims d = SpecDims::of(p.L);
-  const void* g[4];
+  GreenRef g[4];  // full half spectra
   for (int t = 0; t < 4; ++t) {
     if constexpr (sizeof(T) == 4) {
       if (!p.spec32[t].p) {
@@ -288,9 +288,9 @@ int pipe_t(osc_ctx* c, const osc_mesh& m, int direct, const double* rho, double*
         convert_spectrum(c, p.spec64[t].as<cpx<double>>(), p.spec32[t].as<cpx<float>>(),
                          d.size());
       }
-      g[t] = p.spec32[t].p;
+      g[t].g = p.spec32[t].p;
     } else {
-      g[t] = p.spec64[t].p;
+      g[t].g = p.spec64[t].p;
     }
   }
   // ccon = t1 - t2 - t3 + t4 with correlations in y, x, xy (core:790-806)
