@@ -466,9 +466,10 @@ __device__ __forceinline__ void seg_step(uint32_t k, double (&t)[8]) {
 constexpr int kSegSerial = 64;
 
 //
-// Slot s holds particle sv[s]; its record rec[sv[s]] is read here (and, for a
-// following gather, stored to srec[s] with inv[sv[s]] = s) for every slot,
-// the dropped ones at the end included.
+// Slot s holds particle sv[s]; its record rec[sv[s]] is read here (and, with
+// KEEP, for a following gather, stored to srec[s] with inv[sv[s]] = s) for
+// every slot, the dropped ones at the end included.
+template <bool KEEP>
 __global__ void __launch_bounds__(kThreads)
     k_seg_reduce(const uint32_t* __restrict__ ks, const uint32_t* __restrict__ sv,
                  const double4* __restrict__ rec, double4* __restrict__ srec,
@@ -489,8 +490,10 @@ __global__ void __launch_bounds__(kThreads)
     if (i < n) {
       const uint32_t v = sv[i];
       r = rec[v];
-      if (srec) srec[i] = r;
-      if (inv) inv[v] = uint32_t(i);
+      if (KEEP) {
+        srec[i] = r;
+        inv[v] = uint32_t(i);
+      }
     }
     const uint32_t k = valid ? ks[i] : 0xFFFFFFFFu;
     double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -826,10 +829,14 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   }
   if (n > 0) {
     ProfScope ps_cells(c, ST_DEP_CELLS);
-    hipLaunchKernelGGL(k_seg_reduce, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk, sv,
-                       rec.as<const double4>(), keep ? srec.as<double4>() : nullptr,
-                       keep ? inv.as<uint32_t>() : nullptr, n, cnt, cp, S.as<double>(),
-                       head.as<double>(), tail.as<double>());
+    if (keep)
+      hipLaunchKernelGGL(k_seg_reduce<true>, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk, sv,
+                         rec.as<const double4>(), srec.as<double4>(), inv.as<uint32_t>(), n, cnt,
+                         cp, S.as<double>(), head.as<double>(), tail.as<double>());
+    else
+      hipLaunchKernelGGL(k_seg_reduce<false>, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk, sv,
+                         rec.as<const double4>(), nullptr, nullptr, n, cnt, cp, S.as<double>(),
+                         head.as<double>(), tail.as<double>());
     OSC_HIP(hipGetLastError());
     ps_cells.stop();
     ProfScope pf(c, ST_DEP_FIX);
