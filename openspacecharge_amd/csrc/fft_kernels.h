@@ -467,20 +467,29 @@ __device__ __forceinline__ cpx<T> zreal_in(const cpx<T>* gl, int64_t sz, const T
 // is z-real: per (kx, ky) line the Green line is transformed along z here (it
 // is never stored as a full spectrum), multiplied by its rho_hat factor into
 // LDS and transformed back.
+//
+// One-dimensional grid, XCD-aware: workgroups are dealt to the 8 XCDs (each
+// with its own L2) in block-id order, so the outputs of one column unit u
+// get block ids 8*nout*(u/8) + 8*o + u%8 -- one XCD, a few dispatches apart --
+// and the second and later reads of the column's rho_hat line hit that L2.
+// Units are (x block, ky) pairs, nxb x blocks per ky.
 template <typename T, int L>
 __global__ void __launch_bounds__((Geo<T, L>::NT))
     k_zinv_g(const cpx<T>* __restrict__ rho, cpx<T>* __restrict__ out, ZinvIO<T> a,
-             const cpx<T>* __restrict__ tw) {
+             const cpx<T>* __restrict__ tw, int nxb) {
   constexpr int BX = Geo<T, L>::BX;
   using FF = FftTile<T, L, -1, BX, false>;
   using FI = FftTile<T, L, 1, BX, false>;
   static_assert(FF::LDS == FI::LDS, "forward and inverse tiles differ");
   __shared__ cpx<T> lds[FF::LDS];
-  const int o = blockIdx.z, ky = blockIdx.y;
+  const int chunk = blockIdx.x / (8 * a.nout), r = blockIdx.x % (8 * a.nout);
+  const int o = r / 8, u = chunk * 8 + r % 8;
+  if (u >= nxb * a.Ly) return;  // whole workgroup: past the last unit
+  const int xb = u % nxb, ky = u / nxb;
   const Term<T> tm = a.out[o].t[0];
   {
     FF ff;
-    const int kx = blockIdx.x * BX + ff.b;
+    const int kx = xb * BX + ff.b;
     const bool ok = kx < a.nb;
     const cpx<T>* gl = tm.g + (ok ? kx + int64_t(ky) * a.sy : 0);
     ff.load([&](int, int p) { return ok ? zreal_in<T, L>(gl, a.sz, tm, p) : mk<T>(0, 0); });
@@ -497,7 +506,7 @@ __global__ void __launch_bounds__((Geo<T, L>::NT))
   }
   __syncthreads();
   FI fi;
-  const int kx = blockIdx.x * BX + fi.b;
+  const int kx = xb * BX + fi.b;
   const bool ok = kx < a.nb;
   fi.load([&](int b, int p) { return lds[FI::lidx(b, p)]; });
   __syncthreads();
