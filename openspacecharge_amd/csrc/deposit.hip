@@ -118,7 +118,8 @@ __global__ void k_sum_final(const double* __restrict__ part, int nb, double* __r
 
 // tiles of the radix sort: RS_T threads, RS_I elements each, per block; the
 // key kernel covers one tile with KEY_T threads
-constexpr int RS_T = 256, RS_I = 16, RS_TILE = RS_T * RS_I, RS_W = RS_T / 64;
+constexpr int RS_T = 512, RS_I = 16, RS_TILE = RS_T * RS_I, RS_W = RS_T / 64;
+static_assert(RS_T >= 256, "one thread per digit in the scatter prologue");
 constexpr int KEY_T = 1024, KEY_I = RS_TILE / KEY_T;
 
 // ---------------------------------------------------------------------------
@@ -328,21 +329,25 @@ __global__ void __launch_bounds__(RS_T)
   }
   __syncthreads();
   {
-    const int d = threadIdx.x;  // RS_T == 256 digits at most
+    const int d = threadIdx.x;  // threads 0..255 own a digit; all take part in the scans
+    const bool dig = d < 256;
     uint32_t mine = 0;  // this block's count of digit d; wave prefixes in place
+    if (dig)
 #pragma unroll
-    for (int w = 0; w < RS_W; ++w) {
-      const uint32_t c = wcnt[w][d];
-      wcnt[w][d] = mine;
-      mine += c;
-    }
-    const uint32_t tot = d <= int(mask) ? total[d] : 0u;
+      for (int w = 0; w < RS_W; ++w) {
+        const uint32_t c = wcnt[w][d];
+        wcnt[w][d] = mine;
+        mine += c;
+      }
+    const uint32_t tot = dig && d <= int(mask) ? total[d] : 0u;
     const uint32_t dbase = block_scan_incl(tot, scan) - tot;  // first slot of digit d
     const uint32_t start = block_scan_incl(mine, scan) - mine;  // first local slot of digit d
+    if (dig) {
 #pragma unroll
-    for (int w = 0; w < RS_W; ++w) wcnt[w][d] += start;
-    const uint32_t g = d <= int(mask) ? dbase + col[int64_t(d) * nb + blockIdx.x] : 0u;
-    gofs[d] = g - start;  // global slot of local slot j of digit d: gofs[d] + j
+      for (int w = 0; w < RS_W; ++w) wcnt[w][d] += start;
+      const uint32_t g = d <= int(mask) ? dbase + col[int64_t(d) * nb + blockIdx.x] : 0u;
+      gofs[d] = g - start;  // global slot of local slot j of digit d: gofs[d] + j
+    }
   }
   __syncthreads();
 #pragma unroll
