@@ -93,6 +93,7 @@ OSC_API int osc_ctx_destroy(osc_ctx* c) {
     DeviceGuard g(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->stream != c->own) (void)hipStreamSynchronize(c->own);
+    c->kept = SortedParticles{};
     c->gcache.clear();
     for (int t = 0; t < 4; ++t) {
       c->pipe.spec64[t].reset();
@@ -147,6 +148,8 @@ OSC_API int osc_ctx_set_green_cache(osc_ctx* c, int enable, int64_t max_bytes) {
 OSC_API int osc_ctx_clear_caches(osc_ctx* c) {
   return guarded(c, [&] {
     OSC_HIP(hipStreamSynchronize(c->stream));
+    c->kept = SortedParticles{};
+    c->kept_ok = false;
     c->gcache.clear();
     c->pool.release_free();
     return OSC_OK;
@@ -210,7 +213,8 @@ OSC_API int osc_ctx_profile_read(osc_ctx* c, double* ms, int64_t* launches) {
 // deposit_particles body on staged (device) coordinates; rho is device memory
 static int deposit_staged(osc_ctx* c, Staging& st, int64_t n, const double* x, const double* y,
                           const double* z, const double* qa, const double* total_charge,
-                          int resize_mesh, osc_mesh* m, double* rho, SortedParticles* keep) {
+                          int resize_mesh, osc_mesh* m, double* rho, SortedParticles* keep,
+                          unsigned long long* hash = nullptr) {
   int nn[3];
   mesh_dims(m, nn);
   if (resize_mesh) {  // :176-189
@@ -231,6 +235,7 @@ static int deposit_staged(osc_ctx* c, Staging& st, int64_t n, const double* x, c
   a.x = x;
   a.y = y;
   a.z = z;
+  a.hash = hash;
   if (qa) {  // :244-245
     a.q = st.in(qa, size_t(n));
     m->charge = n > 0 ? device_sum(c, a.q, n) : 0.0;
@@ -265,7 +270,15 @@ OSC_API int osc_deposit_particles(osc_ctx* c, int64_t n, const double* xa, const
     const double* z = st.in(za, size_t(n));
     // upstream resizes the mesh, then returns without touching rho (:249-252)
     double* r = (qa || total_charge) ? st.out(rho, size_t(prod3(nn))) : nullptr;
-    const int rc = deposit_staged(c, st, n, x, y, z, qa, total_charge, resize_mesh, m, r, nullptr);
+    // a large deposit keeps its cell order and the content hash of the points
+    // for an interpolate_field at the same points (the usual next call but one)
+    c->kept_ok = false;
+    const bool keep = r && n >= kGatherSortMin;
+    unsigned long long* hash = reinterpret_cast<unsigned long long*>(
+        static_cast<char*>(c->counters) + kHashDeposit);
+    const int rc = deposit_staged(c, st, n, x, y, z, qa, total_charge, resize_mesh, m, r,
+                                  keep ? &c->kept : nullptr, keep ? hash : nullptr);
+    c->kept_ok = keep && rc == OSC_OK;
     if (rc == OSC_OK && n_dropped) *n_dropped = read_counter(c, 0);
     st.finish();
     return rc;

@@ -152,28 +152,90 @@ __device__ __forceinline__ int gather_cell(double x, double mn, double dinv, int
   return i0 < 0 ? 0 : (i0 > nn - 2 ? nn - 2 : i0);
 }
 
+// Content hash of a particle array: the sum mod 2^64 over particles of a mix
+// of each coordinate's bits with the particle's index (the murmur3 64-bit
+// finaliser), so any change of a coordinate or of the order changes it
+// except with probability ~2^-64; the sum is order-free, so per-wave
+// partial sums may be added atomically.
+__device__ __forceinline__ uint64_t fmix64(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+__device__ __forceinline__ uint64_t particle_mix(int64_t i, double x, double y, double z) {
+  const uint64_t s = uint64_t(i) * 0x9e3779b97f4a7c15ull;
+  return fmix64(uint64_t(__double_as_longlong(x)) ^ s) +
+         fmix64(uint64_t(__double_as_longlong(y)) ^ (s + 0x632be59bd9b4e019ull)) +
+         fmix64(uint64_t(__double_as_longlong(z)) ^ (s + 0x85157af5c3a3f5bbull));
+}
+// the workgroup's sum of the threads' partial hashes, stored at part[blockIdx.x]
+// (every thread calls it; at most 16 waves)
+__device__ __forceinline__ void hash_block(uint64_t* part, uint64_t h) {
+  __shared__ uint64_t ws[16];
+  for (int off = 32; off > 0; off >>= 1) h += uint64_t(__shfl_xor((unsigned long long)h, off, 64));
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int w = 0; w < int(blockDim.x + 63) / 64; ++w) t += ws[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_hash_xyz(const double* __restrict__ x, const double* __restrict__ y,
+                           const double* __restrict__ z, int64_t n, uint64_t* __restrict__ part) {
+  uint64_t h = 0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    h += particle_mix(i, x[i], y[i], z[i]);
+  hash_block(part, h);
+}
+
+// the sum of nparts partial hashes (one workgroup)
+__global__ void k_hash_sum(const uint64_t* __restrict__ part, int nparts,
+                           unsigned long long* __restrict__ out) {
+  uint64_t h = 0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) h += part[b];
+  __shared__ uint64_t ws[16];
+  for (int off = 32; off > 0; off >>= 1) h += uint64_t(__shfl_xor((unsigned long long)h, off, 64));
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int w = 0; w < int(blockDim.x + 63) / 64; ++w) t += ws[w];
+    *out = t;
+  }
+}
+
 // cell key of every particle, and its 32-byte record (x, y, z, q) in input
 // order (the sort's payload: one record per particle instead of three
 // scattered coordinates).  Block b takes the sort tile b (RS_TILE particles)
 // and leaves the tile's count of each first-pass digit (key & mask0) in
-// hist[d * nb + b], the layout of k_rs_hist.
+// hist[d * nb + b], the layout of k_rs_hist.  With hpart, the workgroup's
+// partial content hash of (x, y, z) goes to hpart[b].
 __global__ void __launch_bounds__(KEY_T)
     k_cic_keys(const double* __restrict__ x, const double* __restrict__ y,
                const double* __restrict__ z, const double* __restrict__ q, int64_t n, KeyParams p,
                uint32_t* __restrict__ keys, double4* __restrict__ rec,
-               uint32_t* __restrict__ ndrop, uint32_t mask0, uint32_t* __restrict__ hist, int nb) {
+               uint32_t* __restrict__ ndrop, uint32_t mask0, uint32_t* __restrict__ hist, int nb,
+               uint64_t* __restrict__ hpart) {
   __shared__ uint32_t h[256];
   __shared__ uint32_t cnt;
   for (int d = threadIdx.x; d < 256; d += KEY_T) h[d] = 0;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   uint32_t mine = 0;
+  uint64_t hs = 0;
   const int64_t base = int64_t(blockIdx.x) * RS_TILE;
 #pragma unroll
   for (int it = 0; it < KEY_I; ++it) {
     const int64_t i = base + it * KEY_T + threadIdx.x;
     if (i >= n) break;
     const double xi = x[i], yi = y[i], zi = z[i];
+    if (hpart) hs += particle_mix(i, xi, yi, zi);
     int i0, j0, k0;
     bool ok = true;
     if (p.clamp) {
@@ -193,6 +255,7 @@ __global__ void __launch_bounds__(KEY_T)
     atomicAdd(&h[key & mask0], 1u);
     mine += ok ? 0u : 1u;
   }
+  if (hpart) hash_block(hpart, hs);
   if (mine) atomicAdd(&cnt, mine);
   __syncthreads();
   for (int d = threadIdx.x; d <= int(mask0); d += KEY_T) hist[int64_t(d) * nb + blockIdx.x] = h[d];
@@ -765,6 +828,26 @@ static void radix_sort(osc_ctx* c, DevBuf& ka, DevBuf& va, DevBuf& kb, DevBuf& v
   }
 }
 
+bool same_as_deposit(osc_ctx* c, int64_t n, const double* x, const double* y, const double* z) {
+  char* base = static_cast<char*>(c->counters);
+  auto* q = reinterpret_cast<unsigned long long*>(base + kHashQuery);
+  const int nb = int(std::min<int64_t>(nblk(n), 4096));
+  DevBuf part(c->pool, sizeof(uint64_t) * std::max(nb, 1));
+  fill_u32(c, reinterpret_cast<uint32_t*>(q), 2, 0u);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_hash_xyz, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream, x, y, z, n,
+                       part.as<uint64_t>());
+    OSC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_hash_sum, dim3(1), dim3(kThreads), 0, c->stream,
+                       part.as<const uint64_t>(), nb, q);
+    OSC_HIP(hipGetLastError());
+  }
+  unsigned long long h[2] = {0, 0};
+  OSC_HIP(hipMemcpyAsync(h, base + kHashDeposit, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  OSC_HIP(hipStreamSynchronize(c->stream));
+  return h[0] == h[1];
+}
+
 int64_t read_counter(osc_ctx* c, int idx) {
   uint32_t v = 0;
   OSC_HIP(hipMemcpyAsync(&v, static_cast<uint32_t*>(c->counters) + idx, sizeof(v),
@@ -810,10 +893,16 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   DevBuf hist(c->pool, sizeof(uint32_t) * 256 * size_t(sp.nb + 1));
   if (n > 0) {
     ProfScope ps(c, ST_DEP_KEYS);
+    DevBuf hpart(c->pool, a.hash ? sizeof(uint64_t) * sp.nb : 0);
     hipLaunchKernelGGL(k_cic_keys, dim3(sp.nb), dim3(KEY_T), 0, c->stream, a.x, a.y, a.z, a.q, n, kp,
                        ka.as<uint32_t>(), rec.as<double4>(), cnt, sp.mask(0), hist.as<uint32_t>(),
-                       sp.nb);
+                       sp.nb, hpart.as<uint64_t>());
     OSC_HIP(hipGetLastError());
+    if (a.hash) {
+      hipLaunchKernelGGL(k_hash_sum, dim3(1), dim3(kThreads), 0, c->stream,
+                         hpart.as<const uint64_t>(), sp.nb, a.hash);
+      OSC_HIP(hipGetLastError());
+    }
   }
   bool in_b = false;
   DevBuf inv(c->pool, keep ? sizeof(uint32_t) * (n + 1) : 0);
@@ -895,7 +984,8 @@ void sort_for_gather(osc_ctx* c, const InterpArgs& a, int64_t n, const double* x
   if (n > 0) {
     hipLaunchKernelGGL(k_cic_keys, dim3(sp.nb), dim3(KEY_T), 0, c->stream, x, y, z,
                        static_cast<const double*>(nullptr), n, kp, ka.as<uint32_t>(),
-                       rec.as<double4>(), cnt, sp.mask(0), hist.as<uint32_t>(), sp.nb);
+                       rec.as<double4>(), cnt, sp.mask(0), hist.as<uint32_t>(), sp.nb,
+                       static_cast<uint64_t*>(nullptr));
     OSC_HIP(hipGetLastError());
   }
   bool in_b = false;

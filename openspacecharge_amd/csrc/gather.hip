@@ -7,7 +7,7 @@ namespace osc {
 namespace {
 constexpr int kThreads = 256;
 // batches at least this large are sorted by cell before the gather
-constexpr int64_t kSortMin = 1 << 16;
+
 
 struct IP {
   double mn[3], delta[3], hinv[3];
@@ -330,10 +330,22 @@ int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const dou
   }
   uint32_t* cnt = static_cast<uint32_t*>(c->counters) + 2;
   const int64_t nt = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
-  if (n >= kSortMin && n >= nt / 8) {
-    // many points in arbitrary order: sort them by cell first, so that the
+  if (n >= kGatherSortMin && n >= nt / 8) {
+    // many points in arbitrary order: take them in cell order, so that the
     // node records are read with locality (random order reads every node
-    // record line many times over)
+    // record line many times over).  The points of the last large deposit
+    // come with their order (checked by content hash); others are sorted.
+    if (c->kept_ok && c->kept.n == n) {
+      bool same;
+      {
+        ProfScope ps(c, ST_GATHER_SORT);
+        same = same_as_deposit(c, n, x, y, z);
+      }
+      if (same) {
+        interpolate_sorted(c, a, ef, bf, c->kept, E, B, clamped);
+        return 0;
+      }
+    }
     SortedParticles sp;
     {
       ProfScope ps(c, ST_GATHER_SORT);

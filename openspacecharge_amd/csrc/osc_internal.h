@@ -112,6 +112,21 @@ struct ProfRec {
   hipEvent_t a, b;
 };
 
+// particles in cell-sorted order, as left by a deposit (for a following
+// gather) or by sort_for_gather
+struct SortedParticles {
+  int64_t n = 0;
+  DevBuf sv;    // original particle index of each sorted slot (u32)
+  DevBuf srec;  // 32-byte records (x, y, z, q) in sorted order (double4)
+  DevBuf inv;   // sorted slot of each original particle index (u32)
+};
+
+// a gather of at least this many points runs in cell order
+constexpr int64_t kGatherSortMin = int64_t(1) << 16;
+// byte offsets in the context's counter area of the particle-content hashes:
+// of the last osc_deposit_particles, and of a gather's points
+constexpr int kHashDeposit = 64, kHashQuery = 72;
+
 }  // namespace osc
 
 struct osc_ctx {
@@ -124,6 +139,10 @@ struct osc_ctx {
   std::map<int, void*> tw64, tw32;
   osc::GreenCache gcache;
   osc::PipeState pipe;
+  // the cell order of the last osc_deposit_particles (when large), kept for a
+  // following interpolate_field at the same points (content hash checked)
+  osc::SortedParticles kept;
+  bool kept_ok = false;
   void* counters = nullptr;  // small device scratch for counters (persistent)
   void* host_pinned = nullptr;
   bool prof = false;
@@ -263,19 +282,15 @@ struct DepositArgs {
   double q_each;        // uniform charge per particle (q == nullptr, charge mode)
   int harness;          // 1: depose_rho_scalar rules (unit weights, scale at end)
   double scale_after;   // harness chrgpermacro
-};
-// particles in cell-sorted order, as left by a deposit (for a following
-// gather) or by sort_for_gather
-struct SortedParticles {
-  int64_t n = 0;
-  DevBuf sv;    // original particle index of each sorted slot (u32)
-  DevBuf srec;  // 32-byte records (x, y, z, q) in sorted order (double4)
-  DevBuf inv;   // sorted slot of each original particle index (u32)
+  unsigned long long* hash;  // if set: receives the content hash of (x, y, z)
 };
 // rho (nx,ny,nz) on device; the dropped count is left in counter 0
 // (read_counter(c, 0)); keep (optional) receives the sorted particles
 void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles* keep = nullptr);
 int64_t read_counter(osc_ctx* c, int idx);  // synchronises
+// true when the n points (x, y, z) hash to the value left at byte offset
+// kHashDeposit by the last deposit (synchronises)
+bool same_as_deposit(osc_ctx* c, int64_t n, const double* x, const double* y, const double* z);
 void bbox(osc_ctx* c, int64_t n, const double* x, const double* y, const double* z, double lo[3],
           double hi[3]);
 double device_sum(osc_ctx* c, const double* v, int64_t n);

@@ -331,6 +331,40 @@ def test_interpolate_field_bit_exact(ctx, oracle):
     assert np.array_equal(cl, clr) and clr.sum() > 0
 
 
+def test_interpolate_after_deposit_takes_kept_order_only_for_same_points(oracle):
+    """A deposit of >= 65536 points keeps its cell order; interpolate_field at
+    the same points takes it (content hash), at changed points it sorts anew.
+    Either way the fields are bit-exact with the oracle; the stage counts show
+    which way was taken (gather.sort: the hash alone, or hash + sort)."""
+    c = osc.Context(0)
+    x, y, z = gaussian_bunch(70000, (1e-3, 1e-3, 2e-3), 5)
+    m = osc.Mesh3D(nhi=(16, 16, 16), gamma=9.78476)
+    osc.deposit_particles(x, y, z, m, total_charge=1e-9, ctx=c)
+    osc.space_charge_3d(m, calc_bfield=True, ctx=c)
+
+    def check(px, py, pz, scopes):
+        c.profile_read()
+        c.set_profiling(True)
+        (E, B), cl = osc.interpolate_field(px, py, pz, m, E=True, B=True, ctx=c,
+                                           return_clamped=True)
+        st = c.profile_read()
+        c.set_profiling(False)
+        Er, Br, clr = oracle.interpolate_field(m.nlo, m.nhi, m.min, m.delta, m.efield, m.bfield,
+                                               px, py, pz)
+        assert np.array_equal(E, Er.T) and np.array_equal(B, Br.T) and np.array_equal(cl, clr)
+        assert st["gather.sort"][1] == scopes, st
+
+    check(x, y, z, 1)  # the deposit's points: its order
+    x2 = x.copy()
+    x2[123] += 1e-4
+    check(x2, y, z, 2)  # one point moved: sorted anew
+    xs = x.copy()
+    xs[[5, 6]] = xs[[6, 5]]
+    check(xs, y, z, 2)  # two x coordinates swapped
+    check(x, y, z, 1)  # the deposit's points again
+    c.close()
+
+
 # ---------------------------------------------------------------------------
 # FP32 convolution path
 # ---------------------------------------------------------------------------
