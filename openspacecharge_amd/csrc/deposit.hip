@@ -542,8 +542,8 @@ __global__ void __launch_bounds__(kThreads)
     k_seg_reduce(const uint32_t* __restrict__ ks, const uint32_t* __restrict__ sv,
                  const double4* __restrict__ rec, double4* __restrict__ srec,
                  uint32_t* __restrict__ inv, int64_t n, const uint32_t* __restrict__ ndrop,
-                 CellParams p, double* __restrict__ S, double* __restrict__ head,
-                 double* __restrict__ tail) {
+                 CellParams p, double* __restrict__ S, uint8_t* __restrict__ occ,
+                 double* __restrict__ head, double* __restrict__ tail) {
   const int64_t nv = n - int64_t(*ndrop);
   const int64_t ncell = int64_t(p.nn[0]) * p.nn[1] * p.nn[2];
   const int lane = threadIdx.x & 63;
@@ -588,6 +588,7 @@ __global__ void __launch_bounds__(kThreads)
     if (started && ends) {
 #pragma unroll
       for (int c = 0; c < 8; ++c) S[c * ncell + k] = t[c];
+      occ[k] = 1;
     } else if (ends) {
 #pragma unroll
       for (int c = 0; c < 8; ++c) head[w * 8 + c] = t[c];
@@ -631,8 +632,8 @@ __device__ __forceinline__ int64_t run_end_wave(const uint32_t* __restrict__ ks,
 __global__ void k_seg_fix(const uint32_t* __restrict__ ks, int64_t n,
                           const uint32_t* __restrict__ ndrop, int64_t ncell,
                           const double* __restrict__ head, const double* __restrict__ tail,
-                          double* __restrict__ S, uint32_t* __restrict__ heavy,
-                          uint32_t* __restrict__ nheavy) {
+                          double* __restrict__ S, uint8_t* __restrict__ occ,
+                          uint32_t* __restrict__ heavy, uint32_t* __restrict__ nheavy) {
   const int64_t nv = n - int64_t(*ndrop);
   const int64_t nw = (nv + 63) >> 6;
   for (int64_t w = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; w < nw;
@@ -666,6 +667,7 @@ __global__ void k_seg_fix(const uint32_t* __restrict__ ks, int64_t n,
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) S[c * ncell + k] = acc[c] + head[ww * 8 + c];
+    occ[k] = 1;
   }
 }
 
@@ -674,8 +676,8 @@ __global__ void k_seg_fix(const uint32_t* __restrict__ ks, int64_t n,
 __global__ void __launch_bounds__(kThreads)
     k_seg_heavy(const uint32_t* __restrict__ ks, int64_t n, const uint32_t* __restrict__ ndrop,
                 int64_t ncell, const double* __restrict__ head, const double* __restrict__ tail,
-                double* __restrict__ S, const uint32_t* __restrict__ heavy,
-                const uint32_t* __restrict__ nheavy) {
+                double* __restrict__ S, uint8_t* __restrict__ occ,
+                const uint32_t* __restrict__ heavy, const uint32_t* __restrict__ nheavy) {
   __shared__ double red[8][kThreads];
   const int64_t nv = n - int64_t(*ndrop);
   const uint32_t nh = *nheavy;
@@ -699,6 +701,7 @@ __global__ void __launch_bounds__(kThreads)
     if (threadIdx.x < 8) {
       const int c = threadIdx.x;
       S[c * ncell + k] = (tail[w * 8 + c] + red[c][0]) + head[w1 * 8 + c];
+      if (c == 0) occ[k] = 1;
     }
     __syncthreads();
   }
@@ -707,8 +710,10 @@ __global__ void __launch_bounds__(kThreads)
 // ---------------------------------------------------------------------------
 // step 4: nodes
 // ---------------------------------------------------------------------------
-__global__ void k_node(const double* __restrict__ S, int nx, int ny, int nz, int scale_after,
-                       double scale, double* __restrict__ rho) {
+// node = sum of the 8 adjacent cells' corner sums in a fixed order; cells
+// without particles (occ 0) hold no sums and add nothing
+__global__ void k_node(const double* __restrict__ S, const uint8_t* __restrict__ occ, int nx,
+                       int ny, int nz, int scale_after, double scale, double* __restrict__ rho) {
   const int64_t ncell = int64_t(nx) * ny * nz;
   for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < ncell;
        c += int64_t(gridDim.x) * blockDim.x) {
@@ -717,7 +722,10 @@ __global__ void k_node(const double* __restrict__ S, int nx, int ny, int nz, int
 #pragma unroll
     for (int cc = 0; cc < 8; ++cc) {
       const int ci = i - (cc & 1), cj = j - ((cc >> 1) & 1), ck = k - (cc >> 2);
-      if (ci >= 0 && cj >= 0 && ck >= 0) acc = acc + S[cc * ncell + ci + int64_t(nx) * (cj + int64_t(ny) * ck)];
+      if (ci >= 0 && cj >= 0 && ck >= 0) {
+        const int64_t cell = ci + int64_t(nx) * (cj + int64_t(ny) * ck);
+        if (occ[cell]) acc = acc + S[cc * ncell + cell];
+      }
     }
     rho[c] = scale_after ? acc * scale : acc;
   }
@@ -917,36 +925,42 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   const int64_t nw = (n + 63) / 64 + 1;
   DevBuf S(c->pool, sizeof(double) * 8 * ncell), head(c->pool, sizeof(double) * 8 * nw),
       tail(c->pool, sizeof(double) * 8 * nw), heavy(c->pool, sizeof(uint32_t) * nw);
+  // one flag per cell, set where a cell's sums are written (S itself is not
+  // cleared: only flagged cells are read)
+  const int64_t occ_words = (ncell + 3) / 4;
+  DevBuf occ(c->pool, sizeof(uint32_t) * occ_words);
   {
     ProfScope pf(c, ST_DEP_FIX);
-    fill_f64(c, S.as<double>(), 8 * ncell, 0.0);
+    fill_u32(c, occ.as<uint32_t>(), occ_words, 0u);
   }
   if (n > 0) {
     ProfScope ps_cells(c, ST_DEP_CELLS);
     if (keep)
       hipLaunchKernelGGL(k_seg_reduce<true>, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk, sv,
                          rec.as<const double4>(), srec.as<double4>(), inv.as<uint32_t>(), n, cnt,
-                         cp, S.as<double>(), head.as<double>(), tail.as<double>());
+                         cp, S.as<double>(), occ.as<uint8_t>(), head.as<double>(),
+                         tail.as<double>());
     else
       hipLaunchKernelGGL(k_seg_reduce<false>, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk, sv,
                          rec.as<const double4>(), nullptr, nullptr, n, cnt, cp, S.as<double>(),
-                         head.as<double>(), tail.as<double>());
+                         occ.as<uint8_t>(), head.as<double>(), tail.as<double>());
     OSC_HIP(hipGetLastError());
     ps_cells.stop();
     ProfScope pf(c, ST_DEP_FIX);
     hipLaunchKernelGGL(k_seg_fix, dim3(nblk(nw)), dim3(kThreads), 0, c->stream, sk, n, cnt, ncell,
                        head.as<const double>(), tail.as<const double>(), S.as<double>(),
-                       heavy.as<uint32_t>(), cnt + 1);
+                       occ.as<uint8_t>(), heavy.as<uint32_t>(), cnt + 1);
     OSC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_seg_heavy, dim3(256), dim3(kThreads), 0, c->stream, sk, n, cnt, ncell,
                        head.as<const double>(), tail.as<const double>(), S.as<double>(),
-                       heavy.as<const uint32_t>(), cnt + 1);
+                       occ.as<uint8_t>(), heavy.as<const uint32_t>(), cnt + 1);
     OSC_HIP(hipGetLastError());
   }
   {
     ProfScope ps(c, ST_DEP_NODES);
     hipLaunchKernelGGL(k_node, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,
-                       S.as<const double>(), nx, ny, nz, a.harness, a.scale_after, rho);
+                       S.as<const double>(), occ.as<const uint8_t>(), nx, ny, nz, a.harness,
+                       a.scale_after, rho);
     OSC_HIP(hipGetLastError());
   }
   if (keep) {

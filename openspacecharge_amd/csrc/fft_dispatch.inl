@@ -48,10 +48,10 @@ void zinv_t(const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, int Ly, int nout
   constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
   dim3 g = grid3(cdiv(a.nb, BX), Ly, nout);
   if (a.zreal) {
-    const int nxb = int(cdiv(a.nb, BX));
+    const int nxb = int(cdiv(a.nb, ZGeo<T, L>::BX));
     const int64_t units = int64_t(nxb) * Ly;
     const dim3 g1 = grid3(cdiv(units, 8) * 8 * nout, 1, 1);
-    hipLaunchKernelGGL((k_zinv_g<T, L>), g1, dim3(NT), 0, s, rho, out, a, tw, nxb);
+    hipLaunchKernelGGL((k_zinv_g<T, L>), g1, dim3(ZGeo<T, L>::NT), 0, s, rho, out, a, tw, nxb);
   } else
     hipLaunchKernelGGL((k_zinv<T, L>), g, dim3(NT), 0, s, rho, out, a, tw);
 }
