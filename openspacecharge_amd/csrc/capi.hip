@@ -77,8 +77,14 @@ OSC_API int osc_ctx_create(int device, osc_ctx** out) {
   c->device = device;
   DeviceGuard g(device);
   if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&c->counters, 256) != hipSuccess) {
+      hipMalloc(&c->counters, kCounterBytes) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->mesh_host), sizeof(MeshDev),
+                    hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&c->mesh_ev, hipEventDisableTiming) != hipSuccess) {
     (void)hipGetLastError();
+    if (c->mesh_host) (void)hipHostFree(c->mesh_host);
+    if (c->counters) (void)hipFree(c->counters);
+    if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
     return OSC_ERR_HIP;
   }
@@ -108,6 +114,8 @@ OSC_API int osc_ctx_destroy(osc_ctx* c) {
     for (auto& kv : c->tw64) (void)hipFree(kv.second);
     for (auto& kv : c->tw32) (void)hipFree(kv.second);
     if (c->counters) (void)hipFree(c->counters);
+    if (c->mesh_host) (void)hipHostFree(c->mesh_host);
+    if (c->mesh_ev) (void)hipEventDestroy(c->mesh_ev);
     if (c->own) (void)hipStreamDestroy(c->own);
   }
   delete c;
@@ -217,33 +225,38 @@ static int deposit_staged(osc_ctx* c, Staging& st, int64_t n, const double* x, c
                           unsigned long long* hash = nullptr) {
   int nn[3];
   mesh_dims(m, nn);
-  if (resize_mesh) {  // :176-189
-    double lo[3], hi[3], d[3];
-    bbox(c, n, x, y, z, lo, hi);
-    for (int a = 0; a < 3; ++a) {
-      d[a] = (hi[a] - lo[a]) / double(m->nhi[a] - m->nlo[a]);
-      lo[a] = lo[a] - 1.0e-6 * d[a];
-      hi[a] = hi[a] + 1.0e-6 * d[a];
-      d[a] = (hi[a] - lo[a]) / double(m->nhi[a] - m->nlo[a]);
-      m->min[a] = lo[a];
-      m->max[a] = hi[a];
-      m->delta[a] = d[a];
-    }
-  }
+  // The resized mesh (:176-189) and the sum of the charges (:244-245) are
+  // computed on the device; the deposit kernels read the mesh there, and the
+  // host waits for its copy only after they are queued.
+  MeshDev* md = reinterpret_cast<MeshDev*>(static_cast<char*>(c->counters) + kMeshDev);
   DepositArgs a{};
   a.n = n;
   a.x = x;
   a.y = y;
   a.z = z;
   a.hash = hash;
+  bool readback = false;
+  if (resize_mesh) {
+    const int cells[3] = {m->nhi[0] - m->nlo[0], m->nhi[1] - m->nlo[1], m->nhi[2] - m->nlo[2]};
+    bbox_resize(c, n, x, y, z, cells, md);
+    a.md = md;
+    readback = true;
+  }
   if (qa) {  // :244-245
     a.q = st.in(qa, size_t(n));
-    m->charge = n > 0 ? device_sum(c, a.q, n) : 0.0;
+    if (n > 0) {
+      device_sum_to(c, a.q, n, &md->charge);
+      readback = true;
+    } else {
+      m->charge = 0.0;
+    }
   } else if (total_charge) {  // :246-248
     m->charge = *total_charge;
     a.q_each = *total_charge / double(n);
-  } else {
-    return fail(c, OSC_ERR_NO_CHARGE, "Error, must specify qa or total_charge");
+  }
+  if (readback) {
+    OSC_HIP(hipMemcpyAsync(c->mesh_host, md, sizeof(MeshDev), hipMemcpyDeviceToHost, c->stream));
+    OSC_HIP(hipEventRecord(c->mesh_ev, c->stream));
   }
   for (int d = 0; d < 3; ++d) {
     a.nlo[d] = m->nlo[d];
@@ -251,7 +264,21 @@ static int deposit_staged(osc_ctx* c, Staging& st, int64_t n, const double* x, c
     a.mn[d] = m->min[d];
     a.delta[d] = m->delta[d];
   }
-  deposit_cic(c, a, rho, keep);
+  if (qa || total_charge) deposit_cic(c, a, rho, keep);
+  if (readback) {
+    OSC_HIP(hipEventSynchronize(c->mesh_ev));
+    const MeshDev& h = *c->mesh_host;
+    if (resize_mesh)
+      for (int d = 0; d < 3; ++d) {
+        m->min[d] = h.mn[d];
+        m->max[d] = h.mx[d];
+        m->delta[d] = h.d[d];
+      }
+    if (qa) m->charge = h.charge;
+  }
+  // upstream resizes the mesh before it finds no charge (:249-252)
+  if (!qa && !total_charge)
+    return fail(c, OSC_ERR_NO_CHARGE, "Error, must specify qa or total_charge");
   return OSC_OK;
 }
 OSC_API int osc_deposit_particles(osc_ctx* c, int64_t n, const double* xa, const double* ya,

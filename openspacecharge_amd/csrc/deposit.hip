@@ -70,7 +70,18 @@ __global__ void k_bbox_partial(const double* __restrict__ x, const double* __res
   if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = r[threadIdx.x][0];
 }
 
-__global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+// cells of each axis of a resizing deposit (nhi - nlo)
+struct ResizeCells {
+  int n[3];
+};
+
+// Reduces the block partials to (min x, y, z, max x, y, z) in out; with md,
+// also the resized mesh of deposit_particles (open_spacecharge_mod.f90:
+// 176-189), by the same correctly rounded double operations in the same
+// order as the formula there (the library is built with -ffp-contract=off),
+// and the reciprocal spacing the key kernel uses (:225-227).
+__global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __restrict__ out,
+                             MeshDev* __restrict__ md, ResizeCells rc) {
   __shared__ double r[6][kThreads];
   double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   for (int b = threadIdx.x; b < nb; b += kThreads)
@@ -85,6 +96,17 @@ __global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __
     __syncthreads();
   }
   if (threadIdx.x < 6) out[threadIdx.x] = r[threadIdx.x][0];
+  if (md && threadIdx.x < 3) {
+    const int a = threadIdx.x;
+    const double cells = double(rc.n[a]);
+    const double d0 = (r[3 + a][0] - r[a][0]) / cells;
+    const double lo = r[a][0] - 1.0e-6 * d0, hi = r[3 + a][0] + 1.0e-6 * d0;
+    const double d = (hi - lo) / cells;
+    md->mn[a] = lo;
+    md->mx[a] = hi;
+    md->d[a] = d;
+    md->dinv[a] = 1.0 / d;
+  }
 }
 
 __global__ void k_sum_partial(const double* __restrict__ v, int64_t n, double* __restrict__ part) {
@@ -130,6 +152,7 @@ struct KeyParams {
   int nlo[3], nn[3], nhi[3];
   uint32_t sentinel;
   int clamp;  // 0: deposit rule (off-mesh particles get the sentinel); 1: gather rule
+  const MeshDev* md;  // if set: mn and dinv come from here (a mesh resized on the device)
 };
 
 __device__ __forceinline__ bool cell_of(double x, double mn, double dinv, double base, int nlo,
@@ -226,6 +249,11 @@ __global__ void __launch_bounds__(KEY_T)
   __shared__ uint32_t cnt;
   for (int d = threadIdx.x; d < 256; d += KEY_T) h[d] = 0;
   if (threadIdx.x == 0) cnt = 0;
+  if (p.md)
+    for (int d = 0; d < 3; ++d) {
+      p.mn[d] = p.md->mn[d];
+      p.dinv[d] = p.md->dinv[d];
+    }
   __syncthreads();
   uint32_t mine = 0;
   uint64_t hs = 0;
@@ -461,6 +489,7 @@ struct CellParams {
   int hasq;      // per-particle charges
   int harness;   // unit weights (scale applied at the node)
   double q_each;
+  const MeshDev* md;  // if set: mn, d and dinv come from here
 };
 
 // the 8 CIC terms of one particle, corner c = di + 2 dj + 4 dk, each in the
@@ -546,6 +575,12 @@ __global__ void __launch_bounds__(kThreads)
                  double* __restrict__ head, double* __restrict__ tail) {
   const int64_t nv = n - int64_t(*ndrop);
   const int64_t ncell = int64_t(p.nn[0]) * p.nn[1] * p.nn[2];
+  if (p.md)
+    for (int d = 0; d < 3; ++d) {
+      p.mn[d] = p.md->mn[d];
+      p.d[d] = p.md->d[d];
+      p.dinv[d] = p.md->dinv[d];
+    }
   const int lane = threadIdx.x & 63;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   // the loop bound is uniform per block (blockDim is a multiple of 64), so
@@ -745,7 +780,7 @@ void bbox(osc_ctx* c, int64_t n, const double* x, const double* y, const double*
   hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(kThreads), 0, c->stream, x, y, z, n, pp);
   OSC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kThreads), 0, c->stream, pp, nb,
-                     pp + 6 * kRedBlocks);
+                     pp + 6 * kRedBlocks, nullptr, ResizeCells{});
   OSC_HIP(hipGetLastError());
   double h[6];
   OSC_HIP(hipMemcpyAsync(h, pp + 6 * kRedBlocks, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -754,6 +789,28 @@ void bbox(osc_ctx* c, int64_t n, const double* x, const double* y, const double*
     lo[a] = h[a];
     hi[a] = h[3 + a];
   }
+}
+
+void bbox_resize(osc_ctx* c, int64_t n, const double* x, const double* y, const double* z,
+                 const int cells[3], MeshDev* md) {
+  DevBuf part(c->pool, sizeof(double) * 6 * kRedBlocks + 64);
+  double* pp = part.as<double>();
+  int nb = int(std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + kThreads - 1) / kThreads)));
+  ProfScope ps(c, ST_DEP_BBOX);
+  hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(kThreads), 0, c->stream, x, y, z, n, pp);
+  OSC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kThreads), 0, c->stream, pp, nb,
+                     pp + 6 * kRedBlocks, md, ResizeCells{{cells[0], cells[1], cells[2]}});
+  OSC_HIP(hipGetLastError());
+}
+
+void device_sum_to(osc_ctx* c, const double* v, int64_t n, double* dst) {
+  DevBuf part(c->pool, sizeof(double) * kRedBlocks);
+  double* pp = part.as<double>();
+  hipLaunchKernelGGL(k_sum_partial, dim3(kRedBlocks), dim3(kThreads), 0, c->stream, v, n, pp);
+  OSC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kThreads), 0, c->stream, pp, kRedBlocks, dst);
+  OSC_HIP(hipGetLastError());
 }
 
 double device_sum(osc_ctx* c, const double* v, int64_t n) {
@@ -892,6 +949,8 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
     cp.nn[d] = a.nn[d];
   }
   kp.sentinel = uint32_t(ncell);
+  kp.md = a.md;
+  cp.md = a.md;
   cp.hasq = a.q != nullptr;
   cp.harness = a.harness;
   cp.q_each = a.q_each;

@@ -127,6 +127,18 @@ constexpr int64_t kGatherSortMin = int64_t(1) << 16;
 // of the last osc_deposit_particles, and of a gather's points
 constexpr int kHashDeposit = 64, kHashQuery = 72;
 
+// The mesh of a resizing deposit, computed on the device from the bounding
+// box (and the total of the charges when they are given), so that the
+// deposit kernels are queued without a round trip to the host.  It lives at
+// byte offset kMeshDev of the counter area (kCounterBytes long); a copy in
+// pinned host memory fills the caller's mesh once it is needed.
+struct MeshDev {
+  double mn[3], mx[3], d[3], dinv[3];
+  double charge;
+};
+constexpr int kMeshDev = 256, kCounterBytes = 512;
+static_assert(kMeshDev + sizeof(MeshDev) <= kCounterBytes, "counter area too small");
+
 }  // namespace osc
 
 struct osc_ctx {
@@ -144,7 +156,8 @@ struct osc_ctx {
   osc::SortedParticles kept;
   bool kept_ok = false;
   void* counters = nullptr;  // small device scratch for counters (persistent)
-  void* host_pinned = nullptr;
+  osc::MeshDev* mesh_host = nullptr;  // pinned copy of the device mesh
+  hipEvent_t mesh_ev = nullptr;        // recorded after the copy is queued
   bool prof = false;
   std::vector<osc::ProfRec> prof_recs;
   std::vector<hipEvent_t> prof_free;
@@ -283,6 +296,7 @@ struct DepositArgs {
   int harness;          // 1: depose_rho_scalar rules (unit weights, scale at end)
   double scale_after;   // harness chrgpermacro
   unsigned long long* hash;  // if set: receives the content hash of (x, y, z)
+  const MeshDev* md;  // if set: the kernels read mn and delta from here
 };
 // rho (nx,ny,nz) on device; the dropped count is left in counter 0
 // (read_counter(c, 0)); keep (optional) receives the sorted particles
@@ -294,6 +308,13 @@ bool same_as_deposit(osc_ctx* c, int64_t n, const double* x, const double* y, co
 void bbox(osc_ctx* c, int64_t n, const double* x, const double* y, const double* z, double lo[3],
           double hi[3]);
 double device_sum(osc_ctx* c, const double* v, int64_t n);
+// Queued only (no synchronisation): the bounding box of the n points and the
+// resized mesh of deposit_particles (open_spacecharge_mod.f90:176-189) for
+// cells[a] = nhi - nlo mesh cells per axis, into md->mn, mx, d and dinv
+void bbox_resize(osc_ctx* c, int64_t n, const double* x, const double* y, const double* z,
+                 const int cells[3], MeshDev* md);
+// Queued only: the sum of v[0..n) (the order of device_sum) into *dst
+void device_sum_to(osc_ctx* c, const double* v, int64_t n, double* dst);
 
 // ---- gather.hip -------------------------------------------------------------------
 struct InterpArgs {

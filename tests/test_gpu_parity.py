@@ -153,11 +153,15 @@ def test_deposit_heavy_clustering(ctx, oracle):
     assert abs(float(np.sum(m.rho)) - 1e-9) < 1e-20
 
 
-def test_deposit_no_charge(ctx, capsys):
+def test_deposit_no_charge(ctx, oracle, capsys):
     x, y, z = gaussian_bunch(100, (1, 1, 1), seed=1)
     m = osc.Mesh3D(nhi=(8, 8, 8))
     osc.deposit_particles(x, y, z, m, ctx=ctx)
     assert "must specify qa or total_charge" in capsys.readouterr().out
+    # the mesh is resized before the missing charge is found (open_spacecharge_mod.f90:176-189, 249-252)
+    ref = oracle.deposit_particles(x, y, z, (1, 1, 1), (8, 8, 8), total_charge=1.0)
+    assert np.array_equal(m.min, ref["min"]) and np.array_equal(m.max, ref["max"])
+    assert np.array_equal(m.delta, ref["delta"])
 
 
 def test_harness_mesh_and_deposit(ctx, oracle):
