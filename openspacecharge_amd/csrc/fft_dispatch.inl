@@ -47,7 +47,11 @@ void zinv_t(const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, int Ly, int nout
             hipStream_t s) {
   constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
   dim3 g = grid3(cdiv(a.nb, BX), Ly, nout);
-  if (a.zreal) {
+  if (a.zreal && a.zin > 0) {
+    const int nxb = int(cdiv(a.nb, ZGeo<T, L>::BX));
+    hipLaunchKernelGGL((k_zconv<T, L>), grid3(int64_t(nxb) * Ly, 1, 1), dim3(ZGeo<T, L>::NT), 0,
+                       s, rho, out, a, tw, nxb);
+  } else if (a.zreal) {
     const int nxb = int(cdiv(a.nb, ZGeo<T, L>::BX));
     const int64_t units = int64_t(nxb) * Ly;
     const dim3 g1 = grid3(cdiv(units, 8) * 8 * nout, 1, 1);
@@ -150,7 +154,8 @@ void launch_zinv<T>(int L, const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, i
     OSC_FOR_EACH_L(OSC_CASE)
 #undef OSC_CASE
     default: {
-      OSC_REQUIRE(!a.zreal, OSC_ERR_FFT_LENGTH, "z-real Green arrays need a planned length");
+      OSC_REQUIRE(!a.zreal && a.zin == 0, OSC_ERR_FFT_LENGTH,
+                  "z-real Green arrays need a planned length");
       GenericPlan pl = generic_plan(L);
       dim3 g = grid3(cdiv(a.nb, kGenBX), Ly, nout);
       hipLaunchKernelGGL((k_zinv_gen<T>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, rho, out,

@@ -96,6 +96,11 @@ struct ZinvIO {
   // and y only; its z transform is taken per line inside the multiply
   // (k_zinv_g)
   int zreal;
+  // > 0 (with zreal, every output one term with mir == 0): rho_hat has had
+  // its x and y passes only, with zin stored z planes (the rest zero); its z
+  // pass is taken inside the multiply, once per column for all outputs
+  // (k_zconv)
+  int zin;
   OutTerms<T> out[kMaxOut];
 };
 
@@ -526,6 +531,69 @@ __global__ void __launch_bounds__((ZGeo<T, L>::NT))
     const int q = p - a.w0;
     if (ok && q >= 0 && q < a.n_out) ob[q * a.o_sz] = v;
   });
+}
+
+// The forward z pass of rho_hat and, for every output, the z transform of its
+// z-real Green line, the multiply and the inverse z pass, per (kx, ky) column:
+// rho_hat's column (ZinvIO::zin stored planes) is read once and its spectrum
+// kept in registers across the outputs, and the forward and Green spectra
+// meet in registers (same tile, same element positions).  Every output is a
+// single term without mirrors.  One workgroup per (x block, ky) unit.
+//
+// Measured against variants at 256^3: handing the product to the inverse
+// transform by a register permutation instead of through LDS (a thread's
+// output positions are its input positions), with or without reading the
+// next Green line during the inverse transform, needs about 150 VGPRs
+// (3 waves per SIMD instead of 4) and is 5-10 % slower.
+template <typename T, int L>
+__global__ void __launch_bounds__((ZGeo<T, L>::NT))
+    k_zconv(const cpx<T>* __restrict__ rho, cpx<T>* __restrict__ out, ZinvIO<T> a,
+            const cpx<T>* __restrict__ tw, int nxb) {
+  constexpr int BX = ZGeo<T, L>::BX;
+  using FF = FftTile<T, L, -1, BX, false>;
+  using FI = FftTile<T, L, 1, BX, false>;
+  static_assert(FF::LDS == FI::LDS, "forward and inverse tiles differ");
+  __shared__ cpx<T> lds[FF::LDS];
+  const int xb = blockIdx.x % nxb, ky = blockIdx.x / nxb;
+  FF fr;
+  const int kx = xb * BX + fr.b;
+  const bool ok = kx < a.nb;
+  const int64_t col = ok ? kx + int64_t(ky) * a.sy : 0;
+  const cpx<T>* rl = rho + col;
+  fr.load([&](int, int p) { return (ok && p < a.zin) ? rl[p * a.sz] : mk<T>(0, 0); });
+  fr.run(lds, tw);
+#pragma clang loop unroll(disable)
+  for (int o = 0; o < a.nout; ++o) {
+    // the tiles' LDS index arithmetic and twiddle reads stay inside the loop
+    // (hoisted, they would hold two transforms' worth of registers across it)
+    const cpx<T>* twl = tw;
+    asm volatile("" : "+s"(twl));
+    const Term<T> tm = a.out[o].t[0];
+    const cpx<T>* gl = tm.g + col;
+    FF fg;
+    asm volatile("" : "+v"(fg.b), "+v"(fg.t));
+    fg.load([&](int, int p) { return ok ? zreal_in<T, L>(gl, a.sz, tm, p) : mk<T>(0, 0); });
+    __syncthreads();  // the previous transform's last reads of lds
+    fg.run(lds, twl);
+#pragma unroll
+    for (int i = 0; i < FF::E; ++i) {
+      const cpx<T> q = mul(fr.v[i], fg.v[i]);
+      fg.v[i] = mk(tm.sgn * q.x, tm.sgn * q.y);
+    }
+    __syncthreads();  // the transform's last reads of lds
+    fg.dump(lds);
+    __syncthreads();
+    FI fi;
+    asm volatile("" : "+v"(fi.b), "+v"(fi.t));
+    fi.load([&](int b, int p) { return lds[FI::lidx(b, p)]; });
+    __syncthreads();
+    fi.run(lds, twl);
+    cpx<T>* ob = out + o * a.o_so + kx + ky * a.o_sy;
+    fi.store([&](int, int p, cpx<T> v) {
+      const int q = p - a.w0;
+      if (ok && q >= 0 && q < a.n_out) ob[q * a.o_sz] = v;
+    });
+  }
 }
 
 template <typename T>

@@ -145,15 +145,20 @@ void conv_solve(osc_ctx* c, const double* rho, const int n[3], const SpecDims& d
     ProfScope ps(c, ST_FWD_Y);
     launch_axis<T>(L1, -1, sp, sp, ay, nz, 1, twiddles<T>(c, L1), s);
   }
-  AxisIO az{};
-  az.s_b = az.d_b = 1;
-  az.s_p = az.d_p = KXP * L1;
-  az.s_o1 = az.d_o1 = KXP;
-  az.nb = d.KX;
-  az.n_in = nz;
-  az.w0 = 0;
-  az.n_out = L2;
-  {
+  // outputs that are each one unmirrored term over a z-real Green array take
+  // rho_hat's z pass inside the multiply (once per column for all outputs)
+  bool zconv = fft_has_plan(L2);
+  for (const OutSpec& os : outs)
+    zconv = zconv && os.nt == 1 && os.t[0].mir == 0 && os.t[0].g.zr;
+  if (!zconv) {
+    AxisIO az{};
+    az.s_b = az.d_b = 1;
+    az.s_p = az.d_p = KXP * L1;
+    az.s_o1 = az.d_o1 = KXP;
+    az.nb = d.KX;
+    az.n_in = nz;
+    az.w0 = 0;
+    az.n_out = L2;
     ProfScope ps(c, ST_FWD_Z);
     launch_axis<T>(L2, -1, sp, sp, az, L1, 1, twiddles<T>(c, L2), s);
   }
@@ -175,6 +180,7 @@ void conv_solve(osc_ctx* c, const double* rho, const int n[3], const SpecDims& d
   zi.n_out = nz;
   zi.nout = nout;
   zi.zreal = outs[0].t[0].g.zr;
+  zi.zin = zconv ? nz : 0;
   bool multi = false;
   for (int o = 0; o < nout; ++o) {
     zi.out[o].nt = outs[o].nt;
