@@ -245,6 +245,22 @@ struct FftTile {
 #pragma unroll
       for (int r = 0; r < RL; ++r) f(b, t + q * TPL + r * (L / RL), v[q * RL + r]);
   }
+  // load/store with each position split as p = t + c: f receives c, a
+  // compile-time constant once the loops are unrolled (a multiple of TPL)
+  template <class F>
+  __device__ __forceinline__ void load_c(F&& f) {
+#pragma unroll
+    for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) v[q * R0 + r] = f(q * TPL + r * (L / R0));
+  }
+  template <class F>
+  __device__ __forceinline__ void store_c(F&& f) const {
+#pragma unroll
+    for (int q = 0; q < E / RL; ++q)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) f(q * TPL + r * (L / RL), v[q * RL + r]);
+  }
   __device__ __forceinline__ void dump(cpx<T>* lds) const {
 #pragma unroll
     for (int q = 0; q < E / RL; ++q)
@@ -275,7 +291,7 @@ struct FftTile {
       if constexpr (S > 0) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          cpx<T> w = tw[k * r * TWS];
+          cpx<T> w = tw[unsigned(k * r * TWS)];  // unsigned: scalar-base address form
           if (DIR > 0) w.y = -w.y;
           a[r] = mul(a[r], w);
         }

@@ -42,12 +42,23 @@ void xc2r_t(const cpx<T>* in, const C2RIO<T>& a, int nout, const cpx<T>* tw, hip
   dim3 g = grid3(cdiv(cdiv(a.nrows, 2), BX), nout, 1);
   hipLaunchKernelGGL((k_xc2r<T, L>), g, dim3(NT), 0, s, in, a, tw);
 }
+// k_zconv's per-thread byte offsets (line b < BX, position t < TPL of a
+// column block) must fit in 32 bits
+template <int L>
+bool zconv_fits_t(const ZinvIO<T>& a) {
+  const int64_t s = a.sz > a.o_sz ? a.sz : a.o_sz;
+  const int64_t most = (ZGeo<T, L>::BX - 1 + int64_t(PlanInfo<L>::TPL - 1) * s) *
+                       int64_t(sizeof(cpx<T>));
+  return most < (int64_t(1) << 32);
+}
+
 template <int L>
 void zinv_t(const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, int Ly, int nout, const cpx<T>* tw,
             hipStream_t s) {
   constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
   dim3 g = grid3(cdiv(a.nb, BX), Ly, nout);
   if (a.zreal && a.zin > 0) {
+    OSC_REQUIRE(zconv_fits_t<L>(a), OSC_ERR_ARG, "fused z convolution: offsets exceed 32 bits");
     const int nxb = int(cdiv(a.nb, ZGeo<T, L>::BX));
     hipLaunchKernelGGL((k_zconv<T, L>), grid3(int64_t(nxb) * Ly, 1, 1), dim3(ZGeo<T, L>::NT), 0,
                        s, rho, out, a, tw, nxb);
@@ -141,6 +152,19 @@ void launch_xc2r<T>(int L, const cpx<T>* in, const C2RIO<T>& a, int nout, const 
     }
   }
   OSC_HIP(hipGetLastError());
+}
+
+template <>
+bool zconv_fits<T>(int L, const ZinvIO<T>& a) {
+  switch (L) {
+#define OSC_CASE(LL) \
+  case LL:           \
+    return zconv_fits_t<LL>(a);
+    OSC_FOR_EACH_L(OSC_CASE)
+#undef OSC_CASE
+    default:
+      return false;
+  }
 }
 
 template <>

@@ -533,12 +533,23 @@ __global__ void __launch_bounds__((ZGeo<T, L>::NT))
   });
 }
 
+// element at uniform byte address base plus a per-thread 32-bit byte offset
+// (the global_load/store form with a scalar base)
+template <typename T>
+__device__ __forceinline__ const cpx<T>& at_off(const char* base, uint32_t vo) {
+  return *reinterpret_cast<const cpx<T>*>(base + vo);
+}
+
 // The forward z pass of rho_hat and, for every output, the z transform of its
 // z-real Green line, the multiply and the inverse z pass, per (kx, ky) column:
 // rho_hat's column (ZinvIO::zin stored planes) is read once and its spectrum
 // kept in registers across the outputs, and the forward and Green spectra
 // meet in registers (same tile, same element positions).  Every output is a
 // single term without mirrors.  One workgroup per (x block, ky) unit.
+//
+// Addressing: element (line b, position t + c) of a column block is at a
+// uniform base (block, c) plus the per-thread byte offset of (b, t), computed
+// once; the host checks that those offsets fit in 32 bits.
 //
 // Measured against variants at 256^3: handing the product to the inverse
 // transform by a register permutation instead of through LDS (a thread's
@@ -552,29 +563,46 @@ __global__ void __launch_bounds__((ZGeo<T, L>::NT))
   constexpr int BX = ZGeo<T, L>::BX;
   using FF = FftTile<T, L, -1, BX, false>;
   using FI = FftTile<T, L, 1, BX, false>;
+  constexpr int TPL = FF::TPL;
+  constexpr int64_t CB = sizeof(cpx<T>);
   static_assert(FF::LDS == FI::LDS, "forward and inverse tiles differ");
   __shared__ cpx<T> lds[FF::LDS];
   const int xb = blockIdx.x % nxb, ky = blockIdx.x / nxb;
   FF fr;
-  const int kx = xb * BX + fr.b;
-  const bool ok = kx < a.nb;
-  const int64_t col = ok ? kx + int64_t(ky) * a.sy : 0;
-  const cpx<T>* rl = rho + col;
-  fr.load([&](int, int p) { return (ok && p < a.zin) ? rl[p * a.sz] : mk<T>(0, 0); });
+  const int b = fr.b, t = fr.t;
+  const bool ok = xb * BX + b < a.nb;
+  const int64_t col0 = xb * BX + int64_t(ky) * a.sy;  // column of line 0 (uniform)
+  // per-thread byte offsets: (b, t) direct, (b, TPL-1-t) for mirrored reads
+  const uint32_t vo = uint32_t((b + t * a.sz) * CB);
+  const uint32_t vm = uint32_t((b + (TPL - 1 - t) * a.sz) * CB);
+  const uint32_t vout = uint32_t((b + t * a.o_sz) * CB);
+  const char* rb = reinterpret_cast<const char*>(rho + col0);
+  fr.load_c([&](int c) {
+    return (ok && t + c < a.zin) ? at_off<T>(rb + c * a.sz * CB, vo) : mk<T>(0, 0);
+  });
   fr.run(lds, tw);
 #pragma clang loop unroll(disable)
   for (int o = 0; o < a.nout; ++o) {
-    // the tiles' LDS index arithmetic and twiddle reads stay inside the loop
-    // (hoisted, they would hold two transforms' worth of registers across it)
-    const cpx<T>* twl = tw;
-    asm volatile("" : "+s"(twl));
     const Term<T> tm = a.out[o].t[0];
-    const cpx<T>* gl = tm.g + col;
+    const char* gb = reinterpret_cast<const char*>(tm.g + col0);
+    // the tiles' LDS index arithmetic stays inside the loop (hoisted, it would
+    // hold two transforms' worth of addresses in registers across it)
     FF fg;
     asm volatile("" : "+v"(fg.b), "+v"(fg.t));
-    fg.load([&](int, int p) { return ok ? zreal_in<T, L>(gl, a.sz, tm, p) : mk<T>(0, 0); });
+    // element p = t + c of a z-real Green line (zreal_in): mirrored at
+    // p >= zmh to L-2-p = (L-1-TPL-c) + (TPL-1-t), zero at p = L-1
+    fg.load_c([&](int c) {
+      const int p = t + c;
+      if (!ok) return mk<T>(0, 0);
+      if (tm.zmh > 0 && p >= tm.zmh) {
+        if (p == L - 1) return mk<T>(0, 0);
+        const cpx<T> v = at_off<T>(gb + (L - 1 - TPL - c) * a.sz * CB, vm);
+        return tm.zms < 0 ? mk<T>(-v.x, -v.y) : v;
+      }
+      return at_off<T>(gb + c * a.sz * CB, vo);
+    });
     __syncthreads();  // the previous transform's last reads of lds
-    fg.run(lds, twl);
+    fg.run(lds, tw);
 #pragma unroll
     for (int i = 0; i < FF::E; ++i) {
       const cpx<T> q = mul(fr.v[i], fg.v[i]);
@@ -585,13 +613,15 @@ __global__ void __launch_bounds__((ZGeo<T, L>::NT))
     __syncthreads();
     FI fi;
     asm volatile("" : "+v"(fi.b), "+v"(fi.t));
-    fi.load([&](int b, int p) { return lds[FI::lidx(b, p)]; });
+    fi.load([&](int bb, int p) { return lds[FI::lidx(bb, p)]; });
     __syncthreads();
-    fi.run(lds, twl);
-    cpx<T>* ob = out + o * a.o_so + kx + ky * a.o_sy;
-    fi.store([&](int, int p, cpx<T> v) {
-      const int q = p - a.w0;
-      if (ok && q >= 0 && q < a.n_out) ob[q * a.o_sz] = v;
+    fi.run(lds, tw);
+    char* ob = reinterpret_cast<char*>(out + o * a.o_so + col0 - int64_t(ky) * a.sy +
+                                       int64_t(ky) * a.o_sy);
+    fi.store_c([&](int c, cpx<T> v) {
+      const int q = t + c - a.w0;
+      if (ok && q >= 0 && q < a.n_out)
+        *reinterpret_cast<cpx<T>*>(ob + (c - a.w0) * a.o_sz * CB + vout) = v;
     });
   }
 }

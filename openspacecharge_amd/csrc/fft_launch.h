@@ -28,6 +28,10 @@ void launch_xr2c_diff(int L, const double* F, cpx<T>* out, const R2CIO& a, const
 template <typename T>
 void launch_xc2r(int L, const cpx<T>* in, const C2RIO<T>& a, int nout, const cpx<T>* tw,
                  hipStream_t s);
+// true when L has a plan and k_zconv's 32-bit per-thread offsets cover the
+// strides of a (ZinvIO::sz, o_sz)
+template <typename T>
+bool zconv_fits(int L, const ZinvIO<T>& a);
 template <typename T>
 void launch_zinv(int L, const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, int Ly, int nout,
                  const cpx<T>* tw, hipStream_t s);

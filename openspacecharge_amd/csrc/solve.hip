@@ -147,7 +147,9 @@ void conv_solve(osc_ctx* c, const double* rho, const int n[3], const SpecDims& d
   }
   // outputs that are each one unmirrored term over a z-real Green array take
   // rho_hat's z pass inside the multiply (once per column for all outputs)
-  bool zconv = fft_has_plan(L2);
+  ZinvIO<T> zs{};
+  zs.sz = zs.o_sz = KXP * L1;
+  bool zconv = fft_has_plan(L2) && zconv_fits<T>(L2, zs);
   for (const OutSpec& os : outs)
     zconv = zconv && os.nt == 1 && os.t[0].mir == 0 && os.t[0].g.zr;
   if (!zconv) {
