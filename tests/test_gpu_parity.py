@@ -164,6 +164,19 @@ def test_deposit_no_charge(ctx, oracle, capsys):
     assert np.array_equal(m.delta, ref["delta"])
 
 
+def test_deposit_empty_bunch(ctx):
+    # minval/maxval of no particles are +huge/-huge upstream; the resize
+    # (open_spacecharge_mod.f90:176-189) then overflows to min = +inf,
+    # max = -inf, delta = -inf, which the device bounding box (starting from
+    # +-inf) reproduces; no particle is deposited
+    e = np.zeros(0)
+    m = osc.Mesh3D(nhi=(8, 8, 8))
+    nd = osc.deposit_particles(e, e, e, m, total_charge=1e-9, ctx=ctx)
+    assert nd == 0
+    assert np.all(m.min == np.inf) and np.all(m.max == -np.inf) and np.all(m.delta == -np.inf)
+    assert m.rho.shape == (8, 8, 8) and not np.any(m.rho)
+
+
 def test_harness_mesh_and_deposit(ctx, oracle):
     x, y, z = gaussian_bunch(20000, (1e-3, 0.6e-3, 3e-3), seed=3, center=(0, 0, 5e-3))
     for iset in (0, 1):
