@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--particles", type=int, default=NPART)
     p.add_argument("--grid", type=int, default=NX)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--host-steps", type=int, default=3,
+                   help="steps of the three drop-in calls on host arrays (PCIe included); 0 skips")
     p.add_argument("--cpu-timeout", type=float, default=240.0)
     p.add_argument("--cpu-child", nargs=2, metavar=("IN", "OUT"), help=argparse.SUPPRESS)
     p.add_argument("--calibrate-copy", action="store_true",
@@ -170,13 +172,36 @@ def gpu_leg(args, x, y, z):
             step()
         torch.cuda.synchronize()
         tc1 = time.perf_counter()
+        host = None
+        if prec_name == "fp64" and args.host_steps > 0:
+            # the three drop-in calls on host (numpy) arrays, as a Fortran
+            # caller's allocatables arrive: x, y, z cross PCIe in the deposit
+            # and again in the gather; rho, the fields and E, B come back
+            hctx = osc.Context(0, precision=prec)
+            hctx.set_green_cache(False)
+            hmesh = osc.Mesh3D(nhi=(args.grid,) * 3, gamma=GAMMA)
+
+            def step_host():
+                osc.deposit_particles(x, y, z, hmesh, total_charge=QTOT, ctx=hctx)
+                osc.space_charge_3d(hmesh, calc_bfield=True, ctx=hctx)
+                return osc.interpolate_field(x, y, z, hmesh, E=True, B=True, ctx=hctx)
+
+            step_host()
+            th0 = time.perf_counter()
+            for _ in range(args.host_steps):
+                step_host()
+            th1 = time.perf_counter()
+            hctx.close()
+            host = dict(solves_per_s=args.host_steps / (th1 - th0),
+                        ms_per_step=1e3 * (th1 - th0) / args.host_steps, steps=args.host_steps)
+            log(f"host arrays: {host['ms_per_step']:.1f} ms/step")
         fields[prec_name] = (mesh.efield.cpu().numpy().copy(order="F"), mesh.rho.cpu().numpy())
         log(f"{prec_name}: {1e3 * (t1 - t0) / args.steps:.3f} ms/step")
         res[prec_name] = dict(
             solves_per_s=args.steps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / args.steps,
             p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
             cached_green_solves_per_s=args.steps / (tc1 - tc0),
-            three_call_solves_per_s=args.steps / (t31 - t30),
+            three_call_solves_per_s=args.steps / (t31 - t30), host_arrays=host,
             stages_ms={k: round(v[0] / args.steps, 4) for k, v in prof.items()},
             stage_launches={k: v[1] // args.steps for k, v in prof.items()},
             prof=prof, mesh=dict(min=list(mesh.min), max=list(mesh.max), delta=list(mesh.delta)))
@@ -587,6 +612,12 @@ def main():
                                       "fp32": round(res["fp32"]["cached_green_solves_per_s"], 3)},
         "three_call_solves_per_s": {"fp64": round(r64["three_call_solves_per_s"], 3),
                                     "fp32": round(res["fp32"]["three_call_solves_per_s"], 3)},
+        "host_arrays_three_call_fp64": None if not r64["host_arrays"] else {
+            "solves_per_s": round(r64["host_arrays"]["solves_per_s"], 3),
+            "ms_per_step": round(r64["host_arrays"]["ms_per_step"], 2),
+            "steps": r64["host_arrays"]["steps"],
+            "note": "deposit_particles, space_charge_3d, interpolate_field on numpy arrays: "
+                    "PCIe transfers of x, y, z (twice), rho, fields, E and B included"},
         "roofline": {"bound": "hbm", "kernel": f"{kname or dom} (stage {dom})",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_bytes,
