@@ -99,6 +99,7 @@ OSC_API int osc_ctx_destroy(osc_ctx* c) {
     DeviceGuard g(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->stream != c->own) (void)hipStreamSynchronize(c->own);
+    c->ring.reset();
     c->kept = SortedParticles{};
     c->gcache.clear();
     for (int t = 0; t < 4; ++t) {

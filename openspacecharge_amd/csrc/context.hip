@@ -1,6 +1,9 @@
 // Context plumbing: device buffer pool, twiddle tables, pointer staging,
 // FFT length support.
+#include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <thread>
 
 #include "osc_internal.h"
 
@@ -140,6 +143,168 @@ bool fft_len_supported(int L) {
 }
 
 // ---------------------------------------------------------------------------
+// host copies split over a few persistent threads
+// ---------------------------------------------------------------------------
+class CopyThreads {
+ public:
+  explicit CopyThreads(int n) {
+    for (int i = 1; i < n; ++i) th_.emplace_back([this, i] { run(i); });
+  }
+  ~CopyThreads() {
+    {
+      std::lock_guard<std::mutex> l(m_);
+      stop_ = true;
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  // dst[0, bytes) = src[0, bytes); returns when every part is done
+  void copy(void* dst, const void* src, size_t bytes) {
+    const int parts = int(th_.size()) + 1;
+    if (parts == 1 || bytes < (size_t(1) << 20)) {
+      std::memcpy(dst, src, bytes);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> l(m_);
+      dst_ = static_cast<char*>(dst);
+      src_ = static_cast<const char*>(src);
+      bytes_ = bytes;
+      parts_ = parts;
+      left_ = parts - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    part(0);
+    std::unique_lock<std::mutex> l(m_);
+    done_.wait(l, [&] { return left_ == 0; });
+  }
+
+ private:
+  void part(int i) {
+    // 4 KiB-aligned parts, so no two threads touch one page
+    const size_t per = ((bytes_ + parts_ - 1) / parts_ + 4095) & ~size_t(4095);
+    const size_t off = per * size_t(i);
+    if (off < bytes_) std::memcpy(dst_ + off, src_ + off, std::min(per, bytes_ - off));
+  }
+  void run(int i) {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> l(m_);
+        cv_.wait(l, [&] { return gen_ != seen; });
+        seen = gen_;
+        if (stop_) return;
+      }
+      part(i);
+      std::lock_guard<std::mutex> l(m_);
+      if (--left_ == 0) done_.notify_one();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+  char* dst_ = nullptr;
+  const char* src_ = nullptr;
+  size_t bytes_ = 0;
+  int parts_ = 1, left_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// pinned ring for pageable host transfers
+// ---------------------------------------------------------------------------
+HostRing::HostRing() {
+  for (auto& sl : slot_) {
+    OSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.buf), kChunk, hipHostMallocDefault));
+    OSC_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+  }
+  const unsigned hw = std::thread::hardware_concurrency();
+  threads_ = new CopyThreads(int(std::max(1u, std::min(hw, 8u))));
+}
+
+HostRing::~HostRing() {
+  for (auto& sl : slot_) {
+    if (sl.busy) (void)hipEventSynchronize(sl.ev);
+    if (sl.ev) (void)hipEventDestroy(sl.ev);
+    if (sl.buf) (void)hipHostFree(sl.buf);
+  }
+  delete threads_;
+}
+
+// finishes a slot's last copy: waits for its DMA and copies out a to_host chunk
+void HostRing::settle(Slot& sl) {
+  if (!sl.busy) return;
+  OSC_HIP(hipEventSynchronize(sl.ev));
+  sl.busy = false;
+  if (sl.out) threads_->copy(sl.out, sl.buf, sl.len);
+  sl.out = nullptr;
+}
+
+HostRing::Slot& HostRing::take() {
+  Slot& sl = slot_[next_];
+  next_ = (next_ + 1) % kBufs;
+  settle(sl);
+  return sl;
+}
+
+void HostRing::to_device(hipStream_t s, void* dev, const void* host, size_t bytes) {
+  const char* h = static_cast<const char*>(host);
+  char* d = static_cast<char*>(dev);
+  for (size_t off = 0; off < bytes; off += kChunk) {
+    const size_t len = std::min(kChunk, bytes - off);
+    Slot& sl = take();
+    threads_->copy(sl.buf, h + off, len);
+    OSC_HIP(hipMemcpyAsync(d + off, sl.buf, len, hipMemcpyHostToDevice, s));
+    OSC_HIP(hipEventRecord(sl.ev, s));
+    sl.busy = true;
+    sl.seq = ++seq_;
+  }
+}
+
+void HostRing::to_host(hipStream_t s, void* host, const void* dev, size_t bytes) {
+  char* h = static_cast<char*>(host);
+  const char* d = static_cast<const char*>(dev);
+  for (size_t off = 0; off < bytes; off += kChunk) {
+    const size_t len = std::min(kChunk, bytes - off);
+    Slot& sl = take();
+    OSC_HIP(hipMemcpyAsync(sl.buf, d + off, len, hipMemcpyDeviceToHost, s));
+    OSC_HIP(hipEventRecord(sl.ev, s));
+    sl.busy = true;
+    sl.out = h + off;
+    sl.len = len;
+    sl.seq = ++seq_;
+  }
+}
+
+void HostRing::drain() {
+  // oldest first, in the order the copies were queued
+  Slot* order[kBufs];
+  for (int i = 0; i < kBufs; ++i) order[i] = &slot_[i];
+  std::sort(order, order + kBufs, [](const Slot* a, const Slot* b) { return a->seq < b->seq; });
+  for (Slot* sl : order) settle(*sl);
+}
+
+void HostRing::discard() {
+  for (auto& sl : slot_) {
+    if (sl.busy) (void)hipEventSynchronize(sl.ev);
+    sl.busy = false;
+    sl.out = nullptr;
+  }
+}
+
+bool is_pageable(const void* p) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return true;
+  }
+  return at.type == hipMemoryTypeUnregistered;
+}
+
+// ---------------------------------------------------------------------------
 // pointers and staging
 // ---------------------------------------------------------------------------
 bool is_device_ptr(const void* p) {
@@ -153,6 +318,20 @@ bool is_device_ptr(const void* p) {
   return at.type == hipMemoryTypeDevice;
 }
 
+// large pageable arrays go through the context's pinned ring
+static HostRing* ring_for(osc_ctx* c, const void* host, size_t bytes) {
+  if (bytes < HostRing::kMin || !is_pageable(host)) return nullptr;
+  if (!c->ring) c->ring = std::make_unique<HostRing>();
+  return c->ring.get();
+}
+
+static void host_to_device(osc_ctx* c, void* d, const void* p, size_t bytes) {
+  if (HostRing* r = ring_for(c, p, bytes))
+    r->to_device(c->stream, d, p, bytes);
+  else
+    OSC_HIP(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, c->stream));
+}
+
 template <typename U>
 const U* Staging::in(const U* p, size_t n) {
   if (!p || n == 0) return p;
@@ -160,7 +339,7 @@ const U* Staging::in(const U* p, size_t n) {
   any_host_ = true;
   bufs_.emplace_back(c_->pool, n * sizeof(U));
   void* d = bufs_.back().p;
-  OSC_HIP(hipMemcpyAsync(d, p, n * sizeof(U), hipMemcpyHostToDevice, c_->stream));
+  host_to_device(c_, d, p, n * sizeof(U));
   return static_cast<const U*>(d);
 }
 
@@ -171,14 +350,28 @@ U* Staging::out(U* p, size_t n, bool copy_in) {
   any_host_ = true;
   bufs_.emplace_back(c_->pool, n * sizeof(U));
   void* d = bufs_.back().p;
-  if (copy_in) OSC_HIP(hipMemcpyAsync(d, p, n * sizeof(U), hipMemcpyHostToDevice, c_->stream));
+  if (copy_in) host_to_device(c_, d, p, n * sizeof(U));
   outs_.push_back({p, d, n * sizeof(U)});
   return static_cast<U*>(d);
 }
 
+Staging::~Staging() {
+  if (ring_pending_ && c_->ring) c_->ring->discard();
+}
+
 void Staging::finish() {
-  for (auto& o : outs_)
-    OSC_HIP(hipMemcpyAsync(o.host, o.dev, o.bytes, hipMemcpyDeviceToHost, c_->stream));
+  for (auto& o : outs_) {
+    if (HostRing* r = ring_for(c_, o.host, o.bytes)) {
+      ring_pending_ = true;
+      r->to_host(c_->stream, o.host, o.dev, o.bytes);
+    } else {
+      OSC_HIP(hipMemcpyAsync(o.host, o.dev, o.bytes, hipMemcpyDeviceToHost, c_->stream));
+    }
+  }
+  if (ring_pending_) {
+    c_->ring->drain();
+    ring_pending_ = false;
+  }
   if (any_host_ || !c_->async) OSC_HIP(hipStreamSynchronize(c_->stream));
   outs_.clear();
 }
