@@ -172,12 +172,32 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 }
 
+// wsum8 over node records staged in LDS: row r = dj + 2 dk holds the records
+// of nodes q0 + 0..kNodeWin-1 (6 doubles each) of that row offset, q = the
+// cell's position in the rows; same 8 terms in the same order
+constexpr int kNodeWin = 16;
+__device__ __forceinline__ double wsum8_lds(const double* st, int q, int c, double ab, double de,
+                                            double gh) {
+  auto N = [&](int r, int d) { return st[((r * kNodeWin) + q + d) * 6 + c]; };
+  return N(0, 0) * ab * de * gh + N(1, 0) * ab * (1. - de) * gh +
+         N(3, 0) * ab * (1. - de) * (1. - gh) + N(2, 0) * ab * de * (1. - gh) +
+         N(2, 1) * (1. - ab) * de * (1. - gh) + N(3, 1) * (1. - ab) * (1. - de) * (1. - gh) +
+         N(1, 1) * (1. - ab) * (1. - de) * gh + N(0, 1) * (1. - ab) * de * gh;
+}
+
 // cell-ordered gather: slot s (record srec[s]) leaves (E, B, clamped, 0) in
 // the 64-byte record out8[s]; the values are those of k_interp_rec (same
-// terms, same order).  Each wave stages its 64 records in LDS (9-double
-// stride) and stores them as 8 contiguous 512-byte runs: one 64-byte record
-// per lane would make every store instruction touch twice the lines.
+// terms, same order).  When the cells of a wave's 64 particles lie within
+// kNodeWin - 1 consecutive cells of one row (the usual case in cell order),
+// the wave loads the 4 node rows it needs once, one record per lane, into
+// LDS and every lane sums from there: one lane per node record instead of 8
+// record reads per lane (the gather is bound by the vector-memory data path,
+// not HBM).  Other waves read the records directly.  Each wave then stages
+// its 64 results in LDS (9-double stride) and stores them as 8 contiguous
+// 512-byte runs: one 64-byte record per lane would make every store
+// instruction touch twice the lines.
 constexpr int kRecPad = 9;
+static_assert(4 * kNodeWin * 6 <= 64 * kRecPad, "node rows exceed the wave's stage");
 __global__ void __launch_bounds__(kThreads)
     k_interp_cells(const double* __restrict__ R, int wantE, int wantB, int64_t n,
                    const double4* __restrict__ srec, IP p, double* __restrict__ out8,
@@ -196,27 +216,61 @@ __global__ void __launch_bounds__(kThreads)
     const int64_t s = wb + lane;
     double v[6] = {0, 0, 0, 0, 0, 0};
     bool fl = false;
+    double ab = 0, de = 0, gh = 0;
+    int64_t b = 0;
     if (s < n) {
       const double4 r = srec[s];
       const double xi = r.x, yi = r.y, zi = r.z;
       const int ip = clamp_idx(xi, p.mn[0], p.hinv[0], p.nhi[0], fl);
       const int jp = clamp_idx(yi, p.mn[1], p.hinv[1], p.nhi[1], fl);
       const int kp = clamp_idx(zi, p.mn[2], p.hinv[2], p.nhi[2], fl);
-      const double ab = ((p.mn[0] - xi) + ip * p.delta[0]) * p.hinv[0];
-      const double de = ((p.mn[1] - yi) + jp * p.delta[1]) * p.hinv[1];
-      const double gh = ((p.mn[2] - zi) + kp * p.delta[2]) * p.hinv[2];
+      ab = ((p.mn[0] - xi) + ip * p.delta[0]) * p.hinv[0];
+      de = ((p.mn[1] - yi) + jp * p.delta[1]) * p.hinv[1];
+      gh = ((p.mn[2] - zi) + kp * p.delta[2]) * p.hinv[2];
       int i0 = ip - p.nlo[0], j0 = jp - p.nlo[1], k0 = kp - p.nlo[2];
       i0 = i0 < 0 ? 0 : (i0 > p.nn[0] - 2 ? p.nn[0] - 2 : i0);
       j0 = j0 < 0 ? 0 : (j0 > p.nn[1] - 2 ? p.nn[1] - 2 : j0);
       k0 = k0 < 0 ? 0 : (k0 > p.nn[2] - 2 ? p.nn[2] - 2 : k0);
-      const int64_t b = i0 + j0 * sy + k0 * sz;
+      b = i0 + j0 * sy + k0 * sz;
+      mine += fl ? 1u : 0u;
+    }
+    // the wave's cell range (lanes past n take the first lane's cell)
+    const long long b0 = __shfl(static_cast<long long>(b), 0);
+    long long lo = s < n ? static_cast<long long>(b) : b0, hi = lo;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const long long ol = __shfl_xor(lo, off), oh = __shfl_xor(hi, off);
+      lo = ol < lo ? ol : lo;
+      hi = oh > hi ? oh : hi;
+    }
+    if (hi - lo <= kNodeWin - 2) {
+      // lane l loads node lo + (l % kNodeWin) of row l / kNodeWin, up to the
+      // last node the wave's cells use (hi + 1 in each row)
+      const int r = lane / kNodeWin, q = lane % kNodeWin;
+      if (q <= int(hi - lo) + 1) {
+        const int64_t node = lo + q + ((r & 1) ? sy : 0) + ((r & 2) ? sz : 0);
+        const double* src = R + 8 * node;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) st[lane * 6 + c] = src[c];
+      }
+      wave_lds_sync();
+      if (s < n) {
+        const int q0 = int(b - lo);
+        if (wantE)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) v[c] = wsum8_lds(st, q0, c, ab, de, gh);
+        if (wantB)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) v[3 + c] = wsum8_lds(st, q0, 3 + c, ab, de, gh);
+      }
+      wave_lds_sync();  // the node rows are read before the results overwrite them
+    } else if (s < n) {
       if (wantE)
 #pragma unroll
         for (int c = 0; c < 3; ++c) v[c] = wsum8(R, sy, sz, b, c, ab, de, gh);
       if (wantB)
 #pragma unroll
         for (int c = 0; c < 3; ++c) v[3 + c] = wsum8(R, sy, sz, b, 3 + c, ab, de, gh);
-      mine += fl ? 1u : 0u;
     }
 #pragma unroll
     for (int c = 0; c < 6; ++c) st[lane * kRecPad + c] = v[c];
@@ -250,25 +304,44 @@ __global__ void __launch_bounds__(kThreads)
 
 // original order: particle i reads its slot inv[i] (one 64-byte record) and
 // writes E(3,n), B(3,n) and its clamp flag contiguously -- a random read costs
-// one line, while a random 24-byte write costs two partial sectors
+// one line, while a random 24-byte write costs two partial sectors.  Four
+// lanes read one record together (16 bytes each: (E0,E1), (E2,B0), (B1,B2),
+// (flag,pad)), so a load instruction touches 16 lines instead of 64, and each
+// lane stores its part; the 16 particles of one round are consecutive, so
+// the stores of a round cover contiguous runs of E, B and the flags.
 __global__ void k_unpermute(const double* __restrict__ out8, const uint32_t* __restrict__ inv,
                             int64_t n, double* __restrict__ E, double* __restrict__ B,
                             int8_t* __restrict__ clamped) {
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x) {
-    const double4* r = reinterpret_cast<const double4*>(out8 + 8 * int64_t(inv[i]));
-    const double4 a = r[0], b = r[1];
-    if (E) {
-      E[3 * i] = a.x;
-      E[3 * i + 1] = a.y;
-      E[3 * i + 2] = a.z;
+  const int lane = threadIdx.x & 63, part = lane & 3;
+  const int64_t nw = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  // the loop bound is uniform per wave: every lane takes part in the shuffles
+  for (int64_t w = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) >> 6; w * 64 < n; w += nw) {
+    const int64_t i0 = w * 64;
+    const uint32_t s = i0 + lane < n ? inv[i0 + lane] : 0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = 16 * j + (lane >> 2);
+      const uint32_t sr = uint32_t(__shfl(int(s), r));
+      const int64_t i = i0 + r;
+      if (i >= n) continue;
+      const double2 v = reinterpret_cast<const double2*>(out8 + 8 * int64_t(sr))[part];
+      if (part == 0) {
+        if (E) {
+          E[3 * i] = v.x;
+          E[3 * i + 1] = v.y;
+        }
+      } else if (part == 1) {
+        if (E) E[3 * i + 2] = v.x;
+        if (B) B[3 * i] = v.y;
+      } else if (part == 2) {
+        if (B) {
+          B[3 * i + 1] = v.x;
+          B[3 * i + 2] = v.y;
+        }
+      } else if (clamped) {
+        clamped[i] = v.x != 0.0 ? 1 : 0;
+      }
     }
-    if (B) {
-      B[3 * i] = a.w;
-      B[3 * i + 1] = b.x;
-      B[3 * i + 2] = b.y;
-    }
-    if (clamped) clamped[i] = b.z != 0.0 ? 1 : 0;
   }
 }
 }  // namespace
