@@ -85,7 +85,10 @@ def stage_bytes(stage, s, n3, L, P, nout, nspec):
     if stage == "fft.fwd_z":
         return c * KX * L[1] * nz + c * KX * L[1] * L[2]
     if stage == "fft.inv_zmul":
-        return c * KX * L[1] * L[2] * (1 + nspec) + nout * c * KX * L[1] * nz
+        # fused z convolution: rho_hat's nz stored planes once, the z-real
+        # Green arrays (of the three E components two are stored as halves
+        # along z, one whole), each output's nz planes
+        return c * KX * L[1] * nz * (1 + nout) + c * KX * L[1] * 2 * L[2] * nspec // 3
     if stage == "fft.inv_y":
         return nout * (c * KX * L[1] * nz + c * KX * ny * nz)
     if stage == "fft.inv_x":
@@ -95,11 +98,12 @@ def stage_bytes(stage, s, n3, L, P, nout, nspec):
         # kernel of this stage is the one that reads the particles in cell order
         return 24 * P + s * n
     if stage == "gather":
-        # compulsory part of the field gather (the unpermute pass is separate)
+        # compulsory part of the field gather (the return to particle order
+        # is a separate pass)
         return 24 * P + 48 * P + 48 * n
     if stage == "gather.out":
-        # one 64-byte record per particle in, E and B (48 B) out, 4 B index
-        return 64 * P + 48 * P + 4 * P
+        # one 64-byte record per particle in, E and B (48 B) and the flag out
+        return 64 * P + 48 * P + P
     return None
 
 
@@ -446,7 +450,7 @@ def cpu_child(args):
 
 
 # stage of the per-stage HIP-event timing -> the one kernel it times
-STAGE_KERNEL = {"gather": "k_interp_cells", "gather.out": "k_unpermute", "fft.inv_zmul": "k_zinv<double",
+STAGE_KERNEL = {"gather": "k_interp_cells", "gather.out": "k_unpack", "fft.inv_zmul": "k_zconv<double",
                 "fft.inv_x": "k_xc2r<double", "fft.fwd_x": "k_xr2c<double",
                 "deposit.cells": "k_seg_reduce<true>"}
 

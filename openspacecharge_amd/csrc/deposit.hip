@@ -342,7 +342,6 @@ __global__ void __launch_bounds__(RS_T)
 struct SortPayload {
   const double4* rec;
   double4* srec;
-  uint32_t* inv;  // inv[original index] = sorted slot (may be null)
 };
 
 // inclusive scan of one value per thread over the RS_T threads: within each
@@ -461,15 +460,11 @@ __global__ void __launch_bounds__(RS_T)
   }
 }
 
-// records in sorted order (slot s holds particle sv[s]) and, optionally, the
-// inverse permutation
+// records in sorted order (slot s holds particle sv[s])
 __global__ void k_sort_payload(const uint32_t* __restrict__ sv, int64_t n, SortPayload pl) {
   for (int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; s < n;
-       s += int64_t(gridDim.x) * blockDim.x) {
-    const uint32_t v = sv[s];
-    pl.srec[s] = pl.rec[v];
-    if (pl.inv) pl.inv[v] = uint32_t(s);
-  }
+       s += int64_t(gridDim.x) * blockDim.x)
+    pl.srec[s] = pl.rec[sv[s]];
 }
 
 __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
@@ -564,13 +559,13 @@ constexpr int kSegSerial = 64;
 
 //
 // Slot s holds particle sv[s]; its record rec[sv[s]] is read here (and, with
-// KEEP, for a following gather, stored to srec[s] with inv[sv[s]] = s) for
-// every slot, the dropped ones at the end included.
+// KEEP, for a following gather, stored to srec[s]) for every slot, the
+// dropped ones at the end included.
 template <bool KEEP>
 __global__ void __launch_bounds__(kThreads)
     k_seg_reduce(const uint32_t* __restrict__ ks, const uint32_t* __restrict__ sv,
                  const double4* __restrict__ rec, double4* __restrict__ srec,
-                 uint32_t* __restrict__ inv, int64_t n, const uint32_t* __restrict__ ndrop,
+                 int64_t n, const uint32_t* __restrict__ ndrop,
                  CellParams p, double* __restrict__ S, uint8_t* __restrict__ occ,
                  double* __restrict__ head, double* __restrict__ tail) {
   const int64_t nv = n - int64_t(*ndrop);
@@ -593,10 +588,7 @@ __global__ void __launch_bounds__(kThreads)
     if (i < n) {
       const uint32_t v = sv[i];
       r = rec[v];
-      if (KEEP) {
-        srec[i] = r;
-        inv[v] = uint32_t(i);
-      }
+      if (KEEP) srec[i] = r;
     }
     const uint32_t k = valid ? ks[i] : 0xFFFFFFFFu;
     double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -972,7 +964,6 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
     }
   }
   bool in_b = false;
-  DevBuf inv(c->pool, keep ? sizeof(uint32_t) * (n + 1) : 0);
   {
     ProfScope ps(c, ST_DEP_SORT);
     radix_sort(c, ka, va, kb, vb, n, sp, hist, SortPayload{}, in_b);
@@ -996,12 +987,12 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
     ProfScope ps_cells(c, ST_DEP_CELLS);
     if (keep)
       hipLaunchKernelGGL(k_seg_reduce<true>, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk, sv,
-                         rec.as<const double4>(), srec.as<double4>(), inv.as<uint32_t>(), n, cnt,
+                         rec.as<const double4>(), srec.as<double4>(), n, cnt,
                          cp, S.as<double>(), occ.as<uint8_t>(), head.as<double>(),
                          tail.as<double>());
     else
       hipLaunchKernelGGL(k_seg_reduce<false>, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk, sv,
-                         rec.as<const double4>(), nullptr, nullptr, n, cnt, cp, S.as<double>(),
+                         rec.as<const double4>(), nullptr, n, cnt, cp, S.as<double>(),
                          occ.as<uint8_t>(), head.as<double>(), tail.as<double>());
     OSC_HIP(hipGetLastError());
     ps_cells.stop();
@@ -1026,7 +1017,6 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
     keep->n = n;
     keep->sv = std::move(in_b ? vb : va);
     keep->srec = std::move(srec);
-    keep->inv = std::move(inv);
   }
 }
 
@@ -1062,11 +1052,9 @@ void sort_for_gather(osc_ctx* c, const InterpArgs& a, int64_t n, const double* x
     OSC_HIP(hipGetLastError());
   }
   bool in_b = false;
-  DevBuf inv(c->pool, sizeof(uint32_t) * (n + 1));
-  SortPayload pl{rec.as<const double4>(), srec.as<double4>(), inv.as<uint32_t>()};
+  SortPayload pl{rec.as<const double4>(), srec.as<double4>()};
   radix_sort(c, ka, va, kb, vb, n, sp, hist, pl, in_b);
   out->n = n;
-  out->inv = std::move(inv);
   out->sv = std::move(in_b ? vb : va);
   out->srec = std::move(srec);
 }

@@ -185,23 +185,24 @@ __device__ __forceinline__ double wsum8_lds(const double* st, int q, int c, doub
          N(1, 1) * (1. - ab) * (1. - de) * gh + N(0, 1) * (1. - ab) * de * gh;
 }
 
-// cell-ordered gather: slot s (record srec[s]) leaves (E, B, clamped, 0) in
-// the 64-byte record out8[s]; the values are those of k_interp_rec (same
-// terms, same order).  When the cells of a wave's 64 particles lie within
+// cell-ordered gather: slot s (record srec[s], particle sv[s]) leaves
+// (E, B, clamped, 0) in the 64-byte record out8[sv[s]]; the values are those
+// of k_interp_rec (same terms, same order).  When the cells of a wave's 64 particles lie within
 // kNodeWin - 1 consecutive cells of one row (the usual case in cell order),
 // the wave loads the 4 node rows it needs once, one record per lane, into
 // LDS and every lane sums from there: one lane per node record instead of 8
 // record reads per lane (the gather is bound by the vector-memory data path,
 // not HBM).  Other waves read the records directly.  Each wave then stages
-// its 64 results in LDS (9-double stride) and stores them as 8 contiguous
-// 512-byte runs: one 64-byte record per lane would make every store
-// instruction touch twice the lines.
+// its 64 results in LDS (9-double stride) and stores each record with four
+// lanes (16 bytes each) at its particle's original position: a random but
+// whole 64-byte line per record, about as costly as the random read it
+// replaces in the return to particle order, which then streams.
 constexpr int kRecPad = 9;
 static_assert(4 * kNodeWin * 6 <= 64 * kRecPad, "node rows exceed the wave's stage");
 __global__ void __launch_bounds__(kThreads)
     k_interp_cells(const double* __restrict__ R, int wantE, int wantB, int64_t n,
-                   const double4* __restrict__ srec, IP p, double* __restrict__ out8,
-                   uint32_t* __restrict__ nclamp) {
+                   const double4* __restrict__ srec, const uint32_t* __restrict__ sv, IP p,
+                   double* __restrict__ out8, uint32_t* __restrict__ nclamp) {
   __shared__ uint32_t cnt;
   __shared__ double stage[kThreads / 64][64 * kRecPad];
   if (threadIdx.x == 0) cnt = 0;
@@ -276,24 +277,21 @@ __global__ void __launch_bounds__(kThreads)
     for (int c = 0; c < 6; ++c) st[lane * kRecPad + c] = v[c];
     st[lane * kRecPad + 6] = fl ? 1.0 : 0.0;
     st[lane * kRecPad + 7] = 0.0;
+    const uint32_t dst = s < n ? sv[s] : 0u;
     // LDS operations of one wave complete in order; the fences keep the
     // compiler from moving the accesses across (records written above are read
     // by other lanes below, and rewritten in the next chunk)
     wave_lds_sync();
-    double* o = out8 + 8 * wb;
-    double t8[8];
+    // round j: lanes 4k..4k+3 store the 4 parts of record 16 j + k
+    const int part = lane & 3;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int e = q * 64 + lane;  // element of the wave's 512 doubles
-      t8[q] = st[(e >> 3) * kRecPad + (e & 7)];
-    }
-    if (wb + 64 <= n) {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) o[q * 64 + lane] = t8[q];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (wb + ((q * 64 + lane) >> 3) < n) o[q * 64 + lane] = t8[q];
+    for (int j = 0; j < 4; ++j) {
+      const int r = 16 * j + (lane >> 2);
+      const uint32_t d = uint32_t(__shfl(int(dst), r));
+      if (wb + r < n) {
+        const double* sr = st + r * kRecPad + 2 * part;
+        reinterpret_cast<double2*>(out8 + 8 * int64_t(d))[part] = make_double2(sr[0], sr[1]);
+      }
     }
     wave_lds_sync();
   }
@@ -302,46 +300,45 @@ __global__ void __launch_bounds__(kThreads)
   if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
 }
 
-// original order: particle i reads its slot inv[i] (one 64-byte record) and
-// writes E(3,n), B(3,n) and its clamp flag contiguously -- a random read costs
-// one line, while a random 24-byte write costs two partial sectors.  Four
-// lanes read one record together (16 bytes each: (E0,E1), (E2,B0), (B1,B2),
-// (flag,pad)), so a load instruction touches 16 lines instead of 64, and each
-// lane stores its part; the 16 particles of one round are consecutive, so
-// the stores of a round cover contiguous runs of E, B and the flags.
-__global__ void k_unpermute(const double* __restrict__ out8, const uint32_t* __restrict__ inv,
-                            int64_t n, double* __restrict__ E, double* __restrict__ B,
-                            int8_t* __restrict__ clamped) {
-  const int lane = threadIdx.x & 63, part = lane & 3;
-  const int64_t nw = (int64_t(gridDim.x) * blockDim.x) >> 6;
-  // the loop bound is uniform per wave: every lane takes part in the shuffles
-  for (int64_t w = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) >> 6; w * 64 < n; w += nw) {
-    const int64_t i0 = w * 64;
-    const uint32_t s = i0 + lane < n ? inv[i0 + lane] : 0u;
+// original order: the 64-byte records out8[i] = (E0, E1, E2, B0, B1, B2,
+// flag, pad) into E(3,n), B(3,n) and the clamp flags.  Each wave moves 64
+// records through LDS: read as 4 contiguous KB, written as contiguous runs of
+// E and B (512 bytes per store instruction) and 64 flag bytes.
+__global__ void __launch_bounds__(kThreads)
+    k_unpack(const double* __restrict__ out8, int64_t n, double* __restrict__ E,
+             double* __restrict__ B, int8_t* __restrict__ clamped) {
+  __shared__ double stage[kThreads / 64][64 * 8];
+  const int lane = threadIdx.x & 63;
+  double* w = stage[threadIdx.x >> 6];
+  const double2* in2 = reinterpret_cast<const double2*>(out8);
+  // wave-uniform loop over chunks of 64 records
+  for (int64_t i0 = blockIdx.x * int64_t(blockDim.x) + (threadIdx.x & ~63); i0 < n;
+       i0 += int64_t(gridDim.x) * blockDim.x) {
+    const int nrec = n - i0 < 64 ? int(n - i0) : 64;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = 16 * j + (lane >> 2);
-      const uint32_t sr = uint32_t(__shfl(int(s), r));
-      const int64_t i = i0 + r;
-      if (i >= n) continue;
-      const double2 v = reinterpret_cast<const double2*>(out8 + 8 * int64_t(sr))[part];
-      if (part == 0) {
-        if (E) {
-          E[3 * i] = v.x;
-          E[3 * i + 1] = v.y;
-        }
-      } else if (part == 1) {
-        if (E) E[3 * i + 2] = v.x;
-        if (B) B[3 * i] = v.y;
-      } else if (part == 2) {
-        if (B) {
-          B[3 * i + 1] = v.x;
-          B[3 * i + 2] = v.y;
-        }
-      } else if (clamped) {
-        clamped[i] = v.x != 0.0 ? 1 : 0;
+    for (int q = 0; q < 4; ++q) {
+      const int e = lane + 64 * q;  // 16-byte element of the chunk, record e / 4
+      if ((e >> 2) < nrec) {
+        const double2 v = in2[4 * i0 + e];
+        w[2 * e] = v.x;
+        w[2 * e + 1] = v.y;
       }
     }
+    wave_lds_sync();
+    if (E)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int e = lane + 64 * k;  // element of the chunk's E: record e / 3, component e % 3
+        if (e / 3 < nrec) E[3 * i0 + e] = w[(e / 3) * 8 + e % 3];
+      }
+    if (B)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int e = lane + 64 * k;
+        if (e / 3 < nrec) B[3 * i0 + e] = w[(e / 3) * 8 + 3 + e % 3];
+      }
+    if (clamped && lane < nrec) clamped[i0 + lane] = w[lane * 8 + 6] != 0.0 ? 1 : 0;
+    wave_lds_sync();  // read before the next chunk overwrites the stage
   }
 }
 }  // namespace
@@ -377,13 +374,14 @@ void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const
       ProfScope ps(c, ST_GATHER);
       hipLaunchKernelGGL(k_interp_cells, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
                          rec.as<const double>(), int(E && ef), int(B && bf), n,
-                         sp.srec.as<const double4>(), p, out8.as<double>(), cnt);
+                         sp.srec.as<const double4>(), sp.sv.as<const uint32_t>(), p,
+                         out8.as<double>(), cnt);
       OSC_HIP(hipGetLastError());
     }
     ProfScope po(c, ST_GATHER_OUT);
-    hipLaunchKernelGGL(k_unpermute, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
-                       out8.as<const double>(), sp.inv.as<const uint32_t>(), n, (E && ef) ? E : nullptr,
-                       (B && bf) ? B : nullptr, clamped);
+    hipLaunchKernelGGL(k_unpack, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
+                       out8.as<const double>(), n, (E && ef) ? E : nullptr, (B && bf) ? B : nullptr,
+                       clamped);
     OSC_HIP(hipGetLastError());
   }
 }

@@ -118,7 +118,6 @@ struct SortedParticles {
   int64_t n = 0;
   DevBuf sv;    // original particle index of each sorted slot (u32)
   DevBuf srec;  // 32-byte records (x, y, z, q) in sorted order (double4)
-  DevBuf inv;   // sorted slot of each original particle index (u32)
 };
 
 // a gather of at least this many points runs in cell order
