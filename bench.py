@@ -130,8 +130,9 @@ def gpu_leg(args, x, y, z):
                                          ctx=ctx)
 
         def step3():
-            # the same step as three drop-in calls
-            osc.deposit_particles(xs, ys, zs, mesh, total_charge=QTOT, ctx=ctx)
+            # the same step as three drop-in calls (upstream's deposit_particles
+            # returns no dropped count, so none is asked for)
+            osc.deposit_particles(xs, ys, zs, mesh, total_charge=QTOT, ctx=ctx, count_dropped=False)
             osc.space_charge_3d(mesh, calc_bfield=True, ctx=ctx)
             return osc.interpolate_field(xs, ys, zs, mesh, E=True, B=True, ctx=ctx)
 

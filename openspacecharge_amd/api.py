@@ -177,9 +177,11 @@ def print_mesh3d(mesh3d: Mesh3D) -> None:
 # deposition
 # ---------------------------------------------------------------------------
 def deposit_particles(xa, ya, za, mesh3d: Mesh3D, qa=None, total_charge=None, resize_mesh=True,
-                      ctx: Optional[K.Context] = None) -> int:
+                      ctx: Optional[K.Context] = None, count_dropped=True):
     """deposit_particles (open_spacecharge_mod.f90:156-288). Returns the number
-    of particles skipped because their CIC stencil leaves the mesh."""
+    of particles skipped because their CIC stencil leaves the mesh; with
+    count_dropped=False, None (upstream returns no count), so that a call on
+    device arrays in an asynchronous context does not wait for the deposit."""
     ctx = ctx or context()
     kx, px = _vec(xa)
     ky, py = _vec(ya)
@@ -192,13 +194,13 @@ def deposit_particles(xa, ya, za, mesh3d: Mesh3D, qa=None, total_charge=None, re
     nd = C.c_int64(0)
     rc = ctx.lib.osc_deposit_particles(ctx.h, n, px, py, pz, pq, C.byref(tc) if tc is not None else None,
                                        int(bool(resize_mesh)), C.byref(m), _ptr(mesh3d.rho, mesh3d.n),
-                                       C.byref(nd))
+                                       C.byref(nd) if count_dropped else None)
     mesh3d.from_c(m)
     if rc == K.OSC_ERR_NO_CHARGE:
         print(" Error, must specify qa or total_charge")
         return 0
     ctx.check(rc)
-    return int(nd.value)
+    return int(nd.value) if count_dropped else None
 
 
 def space_charge_step(xa, ya, za, mesh3d: Mesh3D, qa=None, total_charge=None, resize_mesh=True,
@@ -487,21 +489,22 @@ def interpolate_field(x, y, z, mesh3d: Mesh3D, E=True, B=False, ctx: Optional[K.
 
     Eo = out(3) if E else None
     Bo = out(3) if B else None
-    if dev:
-        import torch
-        cl = torch.empty(n, dtype=torch.int8, device=kx.device)
-        pcl = cl.data_ptr()
-    else:
-        cl = np.empty(n, dtype=np.int8)
-        pcl = cl.ctypes.data
+    cl, pcl = None, None
+    if return_clamped:
+        if dev:
+            import torch
+            cl = torch.empty(n, dtype=torch.int8, device=kx.device)
+            pcl = cl.data_ptr()
+        else:
+            cl = np.empty(n, dtype=np.int8)
+            pcl = cl.ctypes.data
     m = mesh3d.to_c()
     nt = mesh3d.n
-    ncl = C.c_int64(0)
     pE = (Eo.data_ptr() if dev else Eo.ctypes.data) if Eo is not None else None
     pB = (Bo.data_ptr() if dev else Bo.ctypes.data) if Bo is not None else None
     ctx.check(ctx.lib.osc_interpolate_field(ctx.h, C.byref(m), _ptr(mesh3d.efield, 3 * nt) if E else None,
                                             _ptr(mesh3d.bfield, 3 * nt) if B else None, n, px, py, pz, pE,
-                                            pB, pcl, C.byref(ncl)))
+                                            pB, pcl, None))
     if scalar:
         Eo = Eo[0] if Eo is not None else None
         Bo = Bo[0] if Bo is not None else None
