@@ -739,21 +739,33 @@ __global__ void __launch_bounds__(kThreads)
 // ---------------------------------------------------------------------------
 // node = sum of the 8 adjacent cells' corner sums in a fixed order; cells
 // without particles (occ 0) hold no sums and add nothing
+//
+// The 8 flags and the 8 corner sums of a node are read together and the sums
+// of unflagged cells (memory never written this time) are selected away, so
+// a node costs one round of loads.  The accumulator starts at +0 and a sum of
+// doubles never yields -0 in round-to-nearest, so adding a selected +0 leaves
+// it unchanged: the node values are those of the conditional sum.  Cell
+// counts are below 2^31 (deposit_cic), so the index arithmetic is 32-bit.
 __global__ void k_node(const double* __restrict__ S, const uint8_t* __restrict__ occ, int nx,
                        int ny, int nz, int scale_after, double scale, double* __restrict__ rho) {
-  const int64_t ncell = int64_t(nx) * ny * nz;
-  for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < ncell;
-       c += int64_t(gridDim.x) * blockDim.x) {
-    const int i = int(c % nx), j = int((c / nx) % ny), k = int(c / (int64_t(nx) * ny));
-    double acc = 0.0;
+  const uint32_t ncell = uint32_t(nx) * uint32_t(ny) * uint32_t(nz);
+  const uint32_t sxy = uint32_t(nx) * uint32_t(ny);
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+       c += gridDim.x * blockDim.x) {
+    const uint32_t k = c / sxy, r = c - k * sxy, j = r / uint32_t(nx), i = r - j * uint32_t(nx);
+    uint8_t f[8];
+    double v[8];
 #pragma unroll
     for (int cc = 0; cc < 8; ++cc) {
-      const int ci = i - (cc & 1), cj = j - ((cc >> 1) & 1), ck = k - (cc >> 2);
-      if (ci >= 0 && cj >= 0 && ck >= 0) {
-        const int64_t cell = ci + int64_t(nx) * (cj + int64_t(ny) * ck);
-        if (occ[cell]) acc = acc + S[cc * ncell + cell];
-      }
+      const int di = cc & 1, dj = (cc >> 1) & 1, dk = cc >> 2;
+      const bool in = i >= uint32_t(di) && j >= uint32_t(dj) && k >= uint32_t(dk);
+      const uint32_t cell = c - uint32_t(di) - uint32_t(dj) * uint32_t(nx) - uint32_t(dk) * sxy;
+      f[cc] = in ? occ[cell] : uint8_t(0);
+      v[cc] = in ? S[size_t(cc) * ncell + cell] : 0.0;
     }
+    double acc = 0.0;
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc) acc = acc + (f[cc] ? v[cc] : 0.0);
     rho[c] = scale_after ? acc * scale : acc;
   }
 }
