@@ -101,6 +101,7 @@ void launch_axis<T>(int L, int dir, const cpx<T>* in, cpx<T>* out, const AxisIO&
 template <>
 void launch_xr2c<T>(int L, const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw,
                     hipStream_t s) {
+  OSC_REQUIRE(a.nrows < (int64_t(1) << 31), OSC_ERR_ARG, "x pass: too many rows for 32-bit indices");
   switch (L) {
 #define OSC_CASE(LL)                         \
   case LL:                                   \
@@ -121,6 +122,7 @@ void launch_xr2c<T>(int L, const double* in, cpx<T>* out, const R2CIO& a, const 
 template <>
 void launch_xr2c_diff<T>(int L, const double* F, cpx<T>* out, const R2CIO& a,
                          const CornerDiff& cd, const cpx<T>* tw, hipStream_t s) {
+  OSC_REQUIRE(a.nrows < (int64_t(1) << 31), OSC_ERR_ARG, "x pass: too many rows for 32-bit indices");
   switch (L) {
 #define OSC_CASE(LL)                     \
   case LL:                               \
@@ -137,6 +139,7 @@ void launch_xr2c_diff<T>(int L, const double* F, cpx<T>* out, const R2CIO& a,
 template <>
 void launch_xc2r<T>(int L, const cpx<T>* in, const C2RIO<T>& a, int nout, const cpx<T>* tw,
                     hipStream_t s) {
+  OSC_REQUIRE(a.nrows < (int64_t(1) << 31), OSC_ERR_ARG, "x pass: too many rows for 32-bit indices");
   switch (L) {
 #define OSC_CASE(LL)                  \
   case LL:                            \

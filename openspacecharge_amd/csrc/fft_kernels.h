@@ -190,6 +190,27 @@ __device__ __forceinline__ void r2c_unpack(cpx<T> zk, cpx<T> zm, cpx<T>& A, cpx<
   B = mk(T(0.5) * d.y, T(-0.5) * d.x);
 }
 
+// rows ra (even) and rb = ra + 1 of an x pass as (y, z) = (r % ny_rows,
+// r / ny_rows): one 32-bit division per thread (row counts are below 2^31,
+// checked by the launchers)
+struct RowPair {
+  int ya, za, yb, zb;
+};
+__device__ __forceinline__ RowPair row_pair(int64_t ra, int ny_rows) {
+  RowPair p;
+  const uint32_t r = uint32_t(ra), ny = uint32_t(ny_rows);
+  p.za = int(r / ny);
+  p.ya = int(r - uint32_t(p.za) * ny);
+  if (p.ya + 1 == ny_rows) {
+    p.yb = 0;
+    p.zb = p.za + 1;
+  } else {
+    p.yb = p.ya + 1;
+    p.zb = p.za;
+  }
+  return p;
+}
+
 // The real rows of the x pass taken as the 8-corner difference of a primitive
 // F (open_spacecharge_mod.f90:697-705): row (j, kq) of the result is built
 // from rows (j, k), (j+1, k), (j, k+1), (j+1, k+1) of F with
@@ -201,15 +222,14 @@ struct CornerDiff {
   int L1, L2, sh, zero_raw;
 };
 
-// row r of the corner-difference source: its F row and whether it is interior
+// row (j, kq) of the corner-difference source: its F row and whether it is
+// interior
 __device__ __forceinline__ const double* diff_row(const double* F, const R2CIO& a,
-                                                  const CornerDiff& cd, int64_t r, bool ok,
+                                                  const CornerDiff& cd, int j, int kq, bool ok,
                                                   bool& inner) {
   inner = false;
   if (!ok) return F;
-  const int j = int(r % a.ny_rows);
-  int k = int(r / a.ny_rows) - cd.sh;
-  k %= cd.L2;
+  int k = kq - cd.sh;  // kq < L2 and 0 <= sh < L2: one wrap at most
   if (k < 0) k += cd.L2;
   inner = j < cd.L1 - 1 && k < cd.L2 - 1;
   return F + j * a.in_sy + k * a.in_sz;
@@ -237,18 +257,19 @@ __global__ void __launch_bounds__((Geo<T, L>::NT))
   F f;
   const int64_t ra = 2 * (int64_t(blockIdx.x) * BX + f.b), rb = ra + 1;
   const bool oka = ra < a.nrows, okb = rb < a.nrows;
+  const RowPair rp = row_pair(ra, a.ny_rows);
   if constexpr (DIFF) {
     bool ina, inb;
-    const double* pa = diff_row(in, a, cd, ra, oka, ina);
-    const double* pb = diff_row(in, a, cd, rb, okb, inb);
+    const double* pa = diff_row(in, a, cd, rp.ya, rp.za, oka, ina);
+    const double* pb = diff_row(in, a, cd, rp.yb, rp.zb, okb, inb);
     f.load([&](int, int p) {
       T re = (oka && p < a.n_in) ? T(diff_elem(pa, a, cd, ina, p)) : T(0);
       T im = (okb && p < a.n_in) ? T(diff_elem(pb, a, cd, inb, p)) : T(0);
       return mk(re, im);
     });
   } else {
-    const double* pa = in + (oka ? (ra % a.ny_rows) * a.in_sy + (ra / a.ny_rows) * a.in_sz : 0);
-    const double* pb = in + (okb ? (rb % a.ny_rows) * a.in_sy + (rb / a.ny_rows) * a.in_sz : 0);
+    const double* pa = in + (oka ? int64_t(rp.ya) * a.in_sy + int64_t(rp.za) * a.in_sz : 0);
+    const double* pb = in + (okb ? int64_t(rp.yb) * a.in_sy + int64_t(rp.zb) * a.in_sz : 0);
     f.load([&](int, int p) {
       T re = (oka && p < a.n_in) ? T(pa[p]) : T(0);
       T im = (okb && p < a.n_in) ? T(pb[p]) : T(0);
@@ -259,8 +280,8 @@ __global__ void __launch_bounds__((Geo<T, L>::NT))
   __syncthreads();
   f.dump(lds);
   __syncthreads();
-  cpx<T>* oa = out + (oka ? (ra % a.ny_rows) * a.out_sy + (ra / a.ny_rows) * a.out_sz : 0);
-  cpx<T>* ob = out + (okb ? (rb % a.ny_rows) * a.out_sy + (rb / a.ny_rows) * a.out_sz : 0);
+  cpx<T>* oa = out + (oka ? int64_t(rp.ya) * a.out_sy + int64_t(rp.za) * a.out_sz : 0);
+  cpx<T>* ob = out + (okb ? int64_t(rp.yb) * a.out_sy + int64_t(rp.zb) * a.out_sz : 0);
   for (int k = f.t; k <= L / 2; k += F::TPL) {
     cpx<T> A, B;
     r2c_unpack(lds[F::lidx(f.b, k)], lds[F::lidx(f.b, (L - k) % L)], A, B);
@@ -280,8 +301,9 @@ __global__ void __launch_bounds__(kGenBX* kGenTPL)
   const int L = pl.L;
   const int64_t ra = 2 * (int64_t(blockIdx.x) * kGenBX + f.b), rb = ra + 1;
   const bool oka = ra < a.nrows, okb = rb < a.nrows;
-  const double* pa = in + (oka ? (ra % a.ny_rows) * a.in_sy + (ra / a.ny_rows) * a.in_sz : 0);
-  const double* pb = in + (okb ? (rb % a.ny_rows) * a.in_sy + (rb / a.ny_rows) * a.in_sz : 0);
+  const RowPair rp = row_pair(ra, a.ny_rows);
+  const double* pa = in + (oka ? int64_t(rp.ya) * a.in_sy + int64_t(rp.za) * a.in_sz : 0);
+  const double* pb = in + (okb ? int64_t(rp.yb) * a.in_sy + int64_t(rp.zb) * a.in_sz : 0);
   f.load(lds, [&](int, int p) {
     T re = (oka && p < a.n_in) ? T(pa[p]) : T(0);
     T im = (okb && p < a.n_in) ? T(pb[p]) : T(0);
@@ -289,8 +311,8 @@ __global__ void __launch_bounds__(kGenBX* kGenTPL)
   });
   f.run(pl, tw);
   const int buf = f.result_buf();
-  cpx<T>* oa = out + (oka ? (ra % a.ny_rows) * a.out_sy + (ra / a.ny_rows) * a.out_sz : 0);
-  cpx<T>* ob = out + (okb ? (rb % a.ny_rows) * a.out_sy + (rb / a.ny_rows) * a.out_sz : 0);
+  cpx<T>* oa = out + (oka ? int64_t(rp.ya) * a.out_sy + int64_t(rp.za) * a.out_sz : 0);
+  cpx<T>* ob = out + (okb ? int64_t(rp.yb) * a.out_sy + int64_t(rp.zb) * a.out_sz : 0);
   for (int k = f.t; k <= L / 2; k += kGenTPL) {
     cpx<T> A, B;
     r2c_unpack(lds[f.lidx(f.b, k, buf)], lds[f.lidx(f.b, (L - k) % L, buf)], A, B);
@@ -338,14 +360,15 @@ __global__ void __launch_bounds__((Geo<T, L>::NT))
   const int o = blockIdx.y;
   const int64_t ra = 2 * (int64_t(blockIdx.x) * BX + f.b), rb = ra + 1;
   const bool oka = ra < a.nrows, okb = rb < a.nrows;
+  const RowPair rp = row_pair(ra, a.ny_rows);
   const cpx<T>* base = in + o * a.in_so;
-  const cpx<T>* pa = base + (oka ? (ra % a.ny_rows) * a.in_sy + (ra / a.ny_rows) * a.in_sz : 0);
-  const cpx<T>* pb = base + (okb ? (rb % a.ny_rows) * a.in_sy + (rb / a.ny_rows) * a.in_sz : 0);
+  const cpx<T>* pa = base + (oka ? int64_t(rp.ya) * a.in_sy + int64_t(rp.za) * a.in_sz : 0);
+  const cpx<T>* pb = base + (okb ? int64_t(rp.yb) * a.in_sy + int64_t(rp.zb) * a.in_sz : 0);
   f.load([&](int, int p) { return oka ? c2r_pack(pa, pb, okb, p, L) : mk<T>(0, 0); });
   f.run(lds, tw);
   const Epilogue e = a.ep[o];
-  const int64_t offa = (ra % a.ny_rows) * a.out_sy + (ra / a.ny_rows) * a.out_sz;
-  const int64_t offb = (rb % a.ny_rows) * a.out_sy + (rb / a.ny_rows) * a.out_sz;
+  const int64_t offa = int64_t(rp.ya) * a.out_sy + int64_t(rp.za) * a.out_sz;
+  const int64_t offb = int64_t(rp.yb) * a.out_sy + int64_t(rp.zb) * a.out_sz;
   f.store([&](int, int p, cpx<T> v) {
     const int x = p - a.w0;
     if (x < 0 || x >= a.n_out) return;
@@ -365,14 +388,15 @@ __global__ void __launch_bounds__(kGenBX* kGenTPL)
   const int o = blockIdx.y;
   const int64_t ra = 2 * (int64_t(blockIdx.x) * kGenBX + f.b), rb = ra + 1;
   const bool oka = ra < a.nrows, okb = rb < a.nrows;
+  const RowPair rp = row_pair(ra, a.ny_rows);
   const cpx<T>* base = in + o * a.in_so;
-  const cpx<T>* pa = base + (oka ? (ra % a.ny_rows) * a.in_sy + (ra / a.ny_rows) * a.in_sz : 0);
-  const cpx<T>* pb = base + (okb ? (rb % a.ny_rows) * a.in_sy + (rb / a.ny_rows) * a.in_sz : 0);
+  const cpx<T>* pa = base + (oka ? int64_t(rp.ya) * a.in_sy + int64_t(rp.za) * a.in_sz : 0);
+  const cpx<T>* pb = base + (okb ? int64_t(rp.yb) * a.in_sy + int64_t(rp.zb) * a.in_sz : 0);
   f.load(lds, [&](int, int p) { return oka ? c2r_pack(pa, pb, okb, p, L) : mk<T>(0, 0); });
   f.run(pl, tw);
   const Epilogue e = a.ep[o];
-  const int64_t offa = (ra % a.ny_rows) * a.out_sy + (ra / a.ny_rows) * a.out_sz;
-  const int64_t offb = (rb % a.ny_rows) * a.out_sy + (rb / a.ny_rows) * a.out_sz;
+  const int64_t offa = int64_t(rp.ya) * a.out_sy + int64_t(rp.za) * a.out_sz;
+  const int64_t offb = int64_t(rp.yb) * a.out_sy + int64_t(rp.zb) * a.out_sz;
   f.store([&](int, int p, cpx<T> v) {
     const int x = p - a.w0;
     if (x < 0 || x >= a.n_out) return;
