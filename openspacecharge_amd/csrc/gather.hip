@@ -81,22 +81,48 @@ __global__ void k_interp(const double* __restrict__ ef, const double* __restrict
   __syncthreads();
   if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
 }
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
 // node records: rec[node*RS + 0..2] = E, [3..5] = B (RS = 8, 64-byte records)
-// so that the 8 nodes of a cell are 4 pairs of adjacent records
-__global__ void k_pack_nodes(const double* __restrict__ ef, const double* __restrict__ bf,
-                             int64_t nt, double* __restrict__ rec) {
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nt;
-       i += int64_t(gridDim.x) * blockDim.x) {
-    double r[8];
+// so that the 8 nodes of a cell are 4 pairs of adjacent records.  Each wave
+// reads 64 nodes of the six planar components (512 contiguous bytes per load)
+// and writes their 4 KB of records through LDS as 4 contiguous 1 KB stores.
+__global__ void __launch_bounds__(kThreads)
+    k_pack_nodes(const double* __restrict__ ef, const double* __restrict__ bf, int64_t nt,
+                 double* __restrict__ rec) {
+  __shared__ double stage[kThreads / 64][64 * 9];
+  const int lane = threadIdx.x & 63;
+  double* w = stage[threadIdx.x >> 6];
+  // wave-uniform loop over chunks of 64 nodes
+  for (int64_t i0 = blockIdx.x * int64_t(blockDim.x) + (threadIdx.x & ~63); i0 < nt;
+       i0 += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = i0 + lane;
+    const int nn = nt - i0 < 64 ? int(nt - i0) : 64;
+    if (i < nt) {
+      // record of node i at 9-double stride in LDS (spreads the banks)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      r[c] = ef ? ef[c * nt + i] : 0.0;
-      r[3 + c] = bf ? bf[c * nt + i] : 0.0;
+      for (int c = 0; c < 3; ++c) {
+        w[lane * 9 + c] = ef ? ef[c * nt + i] : 0.0;
+        w[lane * 9 + 3 + c] = bf ? bf[c * nt + i] : 0.0;
+      }
+      w[lane * 9 + 6] = 0.0;
+      w[lane * 9 + 7] = 0.0;
     }
-    r[6] = r[7] = 0.0;
-    double4* o = reinterpret_cast<double4*>(rec + 8 * i);
-    o[0] = make_double4(r[0], r[1], r[2], r[3]);
-    o[1] = make_double4(r[4], r[5], r[6], r[7]);
+    wave_lds_sync();
+    double2* o = reinterpret_cast<double2*>(rec + 8 * i0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = q * 64 + lane;  // 16-byte element of the chunk: record e / 4, part e % 4
+      if ((e >> 2) < nn) {
+        const double* src = w + (e >> 2) * 9 + 2 * (e & 3);
+        o[e] = make_double2(src[0], src[1]);
+      }
+    }
+    wave_lds_sync();  // read before the next chunk overwrites the stage
   }
 }
 
@@ -165,11 +191,6 @@ __global__ void k_interp_rec(const double* __restrict__ R, int wantE, int wantB,
   if (mine) atomicAdd(&cnt, mine);
   __syncthreads();
   if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
-}
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 }
 
 // wsum8 over node records staged in LDS: row r = dj + 2 dk holds the records
