@@ -233,6 +233,13 @@ __global__ void k_hash_sum(const uint64_t* __restrict__ part, int nparts,
   }
 }
 
+// orders a wave's LDS accesses around lanes reading what other lanes wrote
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
 // cell key of every particle, and its 32-byte record (x, y, z, q) in input
 // order (the sort's payload: one record per particle instead of three
 // scattered coordinates).  Block b takes the sort tile b (RS_TILE particles)
@@ -247,6 +254,8 @@ __global__ void __launch_bounds__(KEY_T)
                uint64_t* __restrict__ hpart) {
   __shared__ uint32_t h[256];
   __shared__ uint32_t cnt;
+  // each wave's 64 records, staged to be stored as 2 contiguous KB
+  __shared__ double stage[KEY_T / 64][64 * 5];
   for (int d = threadIdx.x; d < 256; d += KEY_T) h[d] = 0;
   if (threadIdx.x == 0) cnt = 0;
   if (p.md)
@@ -258,30 +267,55 @@ __global__ void __launch_bounds__(KEY_T)
   uint32_t mine = 0;
   uint64_t hs = 0;
   const int64_t base = int64_t(blockIdx.x) * RS_TILE;
+  const int lane = threadIdx.x & 63;
+  double* w = stage[threadIdx.x >> 6];
 #pragma unroll
   for (int it = 0; it < KEY_I; ++it) {
-    const int64_t i = base + it * KEY_T + threadIdx.x;
-    if (i >= n) break;
-    const double xi = x[i], yi = y[i], zi = z[i];
-    if (hpart) hs += particle_mix(i, xi, yi, zi);
-    int i0, j0, k0;
-    bool ok = true;
-    if (p.clamp) {
-      i0 = gather_cell(xi, p.mn[0], p.dinv[0], p.nlo[0], p.nhi[0], p.nn[0]);
-      j0 = gather_cell(yi, p.mn[1], p.dinv[1], p.nlo[1], p.nhi[1], p.nn[1]);
-      k0 = gather_cell(zi, p.mn[2], p.dinv[2], p.nlo[2], p.nhi[2], p.nn[2]);
-    } else {
-      ok = cell_of(xi, p.mn[0], p.dinv[0], p.base[0], p.nlo[0], p.nn[0], i0) &&
-           cell_of(yi, p.mn[1], p.dinv[1], p.base[1], p.nlo[1], p.nn[1], j0) &&
-           cell_of(zi, p.mn[2], p.dinv[2], p.base[2], p.nlo[2], p.nn[2], k0);
+    const int64_t wb = base + it * KEY_T + (threadIdx.x & ~63);  // the wave's first particle
+    if (wb >= n) break;                                          // uniform per wave
+    const int64_t i = wb + lane;
+    double xi = 0.0, yi = 0.0, zi = 0.0, qi = 0.0;
+    if (i < n) {
+      xi = x[i];
+      yi = y[i];
+      zi = z[i];
+      qi = q ? q[i] : 0.0;
+      if (hpart) hs += particle_mix(i, xi, yi, zi);
+      int i0, j0, k0;
+      bool ok = true;
+      if (p.clamp) {
+        i0 = gather_cell(xi, p.mn[0], p.dinv[0], p.nlo[0], p.nhi[0], p.nn[0]);
+        j0 = gather_cell(yi, p.mn[1], p.dinv[1], p.nlo[1], p.nhi[1], p.nn[1]);
+        k0 = gather_cell(zi, p.mn[2], p.dinv[2], p.nlo[2], p.nhi[2], p.nn[2]);
+      } else {
+        ok = cell_of(xi, p.mn[0], p.dinv[0], p.base[0], p.nlo[0], p.nn[0], i0) &&
+             cell_of(yi, p.mn[1], p.dinv[1], p.base[1], p.nlo[1], p.nn[1], j0) &&
+             cell_of(zi, p.mn[2], p.dinv[2], p.base[2], p.nlo[2], p.nn[2], k0);
+      }
+      const uint32_t key =
+          ok ? uint32_t(i0) + uint32_t(p.nn[0]) * (uint32_t(j0) + uint32_t(p.nn[1]) * uint32_t(k0))
+             : p.sentinel;
+      keys[i] = key;
+      atomicAdd(&h[key & mask0], 1u);
+      mine += ok ? 0u : 1u;
     }
-    const uint32_t key =
-        ok ? uint32_t(i0) + uint32_t(p.nn[0]) * (uint32_t(j0) + uint32_t(p.nn[1]) * uint32_t(k0))
-           : p.sentinel;
-    keys[i] = key;
-    rec[i] = make_double4(xi, yi, zi, q ? q[i] : 0.0);
-    atomicAdd(&h[key & mask0], 1u);
-    mine += ok ? 0u : 1u;
+    w[lane * 5] = xi;
+    w[lane * 5 + 1] = yi;
+    w[lane * 5 + 2] = zi;
+    w[lane * 5 + 3] = qi;
+    wave_lds_sync();
+    // 16-byte element e of the wave's records: record e / 2, half e % 2
+    const int nrec = n - wb < 64 ? int(n - wb) : 64;
+    double2* o = reinterpret_cast<double2*>(rec + wb);
+#pragma unroll
+    for (int qq = 0; qq < 2; ++qq) {
+      const int e = qq * 64 + lane;
+      if ((e >> 1) < nrec) {
+        const double* src = w + (e >> 1) * 5 + 2 * (e & 1);
+        o[e] = make_double2(src[0], src[1]);
+      }
+    }
+    wave_lds_sync();  // read before the next iteration overwrites the stage
   }
   if (hpart) hash_block(hpart, hs);
   if (mine) atomicAdd(&cnt, mine);
@@ -362,11 +396,6 @@ __device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t* s) {
 
 // the LDS operations of one wave run in program order; the fences keep the
 // compiler from moving accesses across (other lanes read what was written)
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-}
 
 // One pass over a tile of RS_TILE pairs per block.  Wave w ranks its own
 // contiguous share of the tile (RS_TILE / RS_W pairs, 64 per step, held in
