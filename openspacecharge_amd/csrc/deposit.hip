@@ -614,10 +614,28 @@ __global__ void __launch_bounds__(kThreads)
     const int64_t w = i >> 6;
     const bool valid = i < nv;
     double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (i < n) {
-      const uint32_t v = sv[i];
-      r = rec[v];
-      if (KEEP) srec[i] = r;
+    if (i < n) r = rec[sv[i]];
+    if (KEEP) {
+      // the wave's 64 records in cell order, through LDS as 2 contiguous KB
+      __shared__ double stage[kThreads / 64][64 * 5];
+      double* st = stage[threadIdx.x >> 6];
+      st[lane * 5] = r.x;
+      st[lane * 5 + 1] = r.y;
+      st[lane * 5 + 2] = r.z;
+      st[lane * 5 + 3] = r.w;
+      wave_lds_sync();
+      const int64_t wb = base + (threadIdx.x & ~63);
+      const int nrec = n - wb < 64 ? int(n - wb > 0 ? n - wb : 0) : 64;
+      double2* o = reinterpret_cast<double2*>(srec + wb);
+#pragma unroll
+      for (int qq = 0; qq < 2; ++qq) {
+        const int e = qq * 64 + lane;  // record e / 2, half e % 2
+        if ((e >> 1) < nrec) {
+          const double* src = st + (e >> 1) * 5 + 2 * (e & 1);
+          o[e] = make_double2(src[0], src[1]);
+        }
+      }
+      wave_lds_sync();
     }
     const uint32_t k = valid ? ks[i] : 0xFFFFFFFFu;
     double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
