@@ -613,19 +613,28 @@ __global__ void __launch_bounds__(kThreads)
     const int64_t i = base + threadIdx.x;
     const int64_t w = i >> 6;
     const bool valid = i < nv;
+    // the wave's 64 records: two lanes read each 32-byte record (16 bytes
+    // each, one line per record and load instruction) into LDS
+    __shared__ double stage[kThreads / 64][64 * 5];
+    double* st = stage[threadIdx.x >> 6];
+    const int64_t wb = base + (threadIdx.x & ~63);
+    const int nrec = n - wb < 64 ? int(n - wb > 0 ? n - wb : 0) : 64;
+    const uint32_t myv = i < n ? sv[i] : 0u;
+#pragma unroll
+    for (int qq = 0; qq < 2; ++qq) {
+      const int e = qq * 64 + lane;  // record e / 2, half e % 2
+      const uint32_t vr = uint32_t(__shfl(int(myv), e >> 1));
+      if ((e >> 1) < nrec) {
+        const double2 h = reinterpret_cast<const double2*>(rec + vr)[e & 1];
+        st[(e >> 1) * 5 + 2 * (e & 1)] = h.x;
+        st[(e >> 1) * 5 + 2 * (e & 1) + 1] = h.y;
+      }
+    }
+    wave_lds_sync();
     double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (i < n) r = rec[sv[i]];
+    if (i < n) r = make_double4(st[lane * 5], st[lane * 5 + 1], st[lane * 5 + 2], st[lane * 5 + 3]);
     if (KEEP) {
-      // the wave's 64 records in cell order, through LDS as 2 contiguous KB
-      __shared__ double stage[kThreads / 64][64 * 5];
-      double* st = stage[threadIdx.x >> 6];
-      st[lane * 5] = r.x;
-      st[lane * 5 + 1] = r.y;
-      st[lane * 5 + 2] = r.z;
-      st[lane * 5 + 3] = r.w;
-      wave_lds_sync();
-      const int64_t wb = base + (threadIdx.x & ~63);
-      const int nrec = n - wb < 64 ? int(n - wb > 0 ? n - wb : 0) : 64;
+      // the same records in cell order, stored as 2 contiguous KB
       double2* o = reinterpret_cast<double2*>(srec + wb);
 #pragma unroll
       for (int qq = 0; qq < 2; ++qq) {
@@ -635,8 +644,8 @@ __global__ void __launch_bounds__(kThreads)
           o[e] = make_double2(src[0], src[1]);
         }
       }
-      wave_lds_sync();
     }
+    wave_lds_sync();  // read before the next chunk overwrites the stage
     const uint32_t k = valid ? ks[i] : 0xFFFFFFFFu;
     double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (valid) {
