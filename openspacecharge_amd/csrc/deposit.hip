@@ -140,7 +140,7 @@ __global__ void k_sum_final(const double* __restrict__ part, int nb, double* __r
 
 // tiles of the radix sort: RS_T threads, RS_I elements each, per block; the
 // key kernel covers one tile with KEY_T threads
-constexpr int RS_T = 512, RS_I = 16, RS_TILE = RS_T * RS_I, RS_W = RS_T / 64;
+constexpr int RS_T = 512, RS_I = 10, RS_TILE = RS_T * RS_I, RS_W = RS_T / 64;
 static_assert(RS_T >= 256, "one thread per digit in the scatter prologue");
 constexpr int KEY_T = 1024, KEY_I = RS_TILE / KEY_T;
 
