@@ -78,16 +78,20 @@ OSC_API int osc_ctx_create(int device, osc_ctx** out) {
   DeviceGuard g(device);
   if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->counters, kCounterBytes) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&c->mesh_host), sizeof(MeshDev),
+      hipHostMalloc(reinterpret_cast<void**>(&c->mesh_host), kPinnedBytes,
                     hipHostMallocDefault) != hipSuccess ||
-      hipEventCreateWithFlags(&c->mesh_ev, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->mesh_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->hash_ev, hipEventDisableTiming) != hipSuccess) {
     (void)hipGetLastError();
+    if (c->mesh_ev) (void)hipEventDestroy(c->mesh_ev);
     if (c->mesh_host) (void)hipHostFree(c->mesh_host);
     if (c->counters) (void)hipFree(c->counters);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
     return OSC_ERR_HIP;
   }
+  c->hash_host = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(c->mesh_host) +
+                                                       kHashHost);
   c->stream = c->own;
   *out = c;
   return OSC_OK;
@@ -117,6 +121,7 @@ OSC_API int osc_ctx_destroy(osc_ctx* c) {
     if (c->counters) (void)hipFree(c->counters);
     if (c->mesh_host) (void)hipHostFree(c->mesh_host);
     if (c->mesh_ev) (void)hipEventDestroy(c->mesh_ev);
+    if (c->hash_ev) (void)hipEventDestroy(c->hash_ev);
     if (c->own) (void)hipStreamDestroy(c->own);
   }
   delete c;

@@ -953,7 +953,8 @@ static void radix_sort(osc_ctx* c, DevBuf& ka, DevBuf& va, DevBuf& kb, DevBuf& v
   }
 }
 
-bool same_as_deposit(osc_ctx* c, int64_t n, const double* x, const double* y, const double* z) {
+void queue_deposit_hash_check(osc_ctx* c, int64_t n, const double* x, const double* y,
+                              const double* z) {
   char* base = static_cast<char*>(c->counters);
   auto* q = reinterpret_cast<unsigned long long*>(base + kHashQuery);
   const int nb = int(std::min<int64_t>(nblk(n), 4096));
@@ -967,10 +968,15 @@ bool same_as_deposit(osc_ctx* c, int64_t n, const double* x, const double* y, co
                        part.as<const uint64_t>(), nb, q);
     OSC_HIP(hipGetLastError());
   }
-  unsigned long long h[2] = {0, 0};
-  OSC_HIP(hipMemcpyAsync(h, base + kHashDeposit, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  OSC_HIP(hipStreamSynchronize(c->stream));
-  return h[0] == h[1];
+  static_assert(kHashQuery == kHashDeposit + 8, "hashes not adjacent");
+  OSC_HIP(hipMemcpyAsync(c->hash_host, base + kHashDeposit, 2 * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c->stream));
+  OSC_HIP(hipEventRecord(c->hash_ev, c->stream));
+}
+
+bool deposit_hash_matches(osc_ctx* c) {
+  OSC_HIP(hipEventSynchronize(c->hash_ev));
+  return c->hash_host[0] == c->hash_host[1];
 }
 
 int64_t read_counter(osc_ctx* c, int idx) {

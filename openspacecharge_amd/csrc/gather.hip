@@ -365,8 +365,18 @@ __global__ void __launch_bounds__(kThreads)
 }
 }  // namespace
 
+void pack_nodes(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf, double* rec) {
+  const int64_t nt = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
+  const int64_t nb = std::min<int64_t>((nt + kThreads - 1) / kThreads, 65536 * 8);
+  ProfScope pp(c, ST_GATHER_PACK);
+  hipLaunchKernelGGL(k_pack_nodes, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream, ef, bf, nt,
+                     rec);
+  OSC_HIP(hipGetLastError());
+}
+
 void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
-                        const SortedParticles& sp, double* E, double* B, int8_t* clamped) {
+                        const SortedParticles& sp, double* E, double* B, int8_t* clamped,
+                        const double* packed) {
   IP p{};
   for (int d = 0; d < 3; ++d) {
     p.mn[d] = a.mn[d];
@@ -380,22 +390,19 @@ void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const
   const int64_t nt = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
   uint32_t* cnt = static_cast<uint32_t*>(c->counters) + 2;
   fill_u32(c, cnt, 1, 0u);
-  DevBuf rec(c->pool, sizeof(double) * 8 * nt);
-  int64_t nb = std::min<int64_t>((nt + kThreads - 1) / kThreads, 65536 * 8);
-  {
-    ProfScope pp(c, ST_GATHER_PACK);
-    hipLaunchKernelGGL(k_pack_nodes, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream, ef, bf, nt,
-                       rec.as<double>());
-    OSC_HIP(hipGetLastError());
+  DevBuf rec(c->pool, packed ? 0 : sizeof(double) * 8 * nt);
+  if (!packed) {
+    pack_nodes(c, a, ef, bf, rec.as<double>());
+    packed = rec.as<const double>();
   }
   const int64_t n = sp.n;
   if (n > 0) {
     DevBuf out8(c->pool, sizeof(double) * 8 * n);
-    nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
+    const int64_t nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
     {
       ProfScope ps(c, ST_GATHER);
       hipLaunchKernelGGL(k_interp_cells, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
-                         rec.as<const double>(), int(E && ef), int(B && bf), n,
+                         packed, int(E && ef), int(B && bf), n,
                          sp.srec.as<const double4>(), sp.sv.as<const uint32_t>(), p,
                          out8.as<double>(), cnt);
       OSC_HIP(hipGetLastError());
@@ -429,15 +436,25 @@ int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const dou
     // record line many times over).  The points of the last large deposit
     // come with their order (checked by content hash); others are sorted.
     if (c->kept_ok && c->kept.n == n) {
-      bool same;
       {
         ProfScope ps(c, ST_GATHER_SORT);
-        same = same_as_deposit(c, n, x, y, z);
+        queue_deposit_hash_check(c, n, x, y, z);
       }
-      if (same) {
-        interpolate_sorted(c, a, ef, bf, c->kept, E, B, clamped);
+      // the node records do not depend on the order: packed while the host
+      // waits for the hashes
+      DevBuf rec(c->pool, sizeof(double) * 8 * nt);
+      pack_nodes(c, a, ef, bf, rec.as<double>());
+      if (deposit_hash_matches(c)) {
+        interpolate_sorted(c, a, ef, bf, c->kept, E, B, clamped, rec.as<const double>());
         return 0;
       }
+      SortedParticles sp;
+      {
+        ProfScope ps(c, ST_GATHER_SORT);
+        sort_for_gather(c, a, n, x, y, z, &sp);
+      }
+      interpolate_sorted(c, a, ef, bf, sp, E, B, clamped, rec.as<const double>());
+      return 0;
     }
     SortedParticles sp;
     {

@@ -137,6 +137,9 @@ struct MeshDev {
 };
 constexpr int kMeshDev = 256, kCounterBytes = 512;
 static_assert(kMeshDev + sizeof(MeshDev) <= kCounterBytes, "counter area too small");
+// the context's pinned host block: the mesh copy at 0, the two hashes at kHashHost
+constexpr int kHashHost = 128, kPinnedBytes = 256;
+static_assert(sizeof(MeshDev) <= kHashHost, "pinned block too small");
 
 }  // namespace osc
 
@@ -162,6 +165,10 @@ struct osc_ctx {
   void* counters = nullptr;  // small device scratch for counters (persistent)
   osc::MeshDev* mesh_host = nullptr;  // pinned copy of the device mesh
   hipEvent_t mesh_ev = nullptr;        // recorded after the copy is queued
+  // pinned copies of the deposit's and the query's content hashes (in the
+  // mesh_host allocation), and the event recorded after their copy
+  unsigned long long* hash_host = nullptr;
+  hipEvent_t hash_ev = nullptr;
   bool prof = false;
   std::vector<osc::ProfRec> prof_recs;
   std::vector<hipEvent_t> prof_free;
@@ -353,9 +360,13 @@ struct DepositArgs {
 // (read_counter(c, 0)); keep (optional) receives the sorted particles
 void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles* keep = nullptr);
 int64_t read_counter(osc_ctx* c, int idx);  // synchronises
-// true when the n points (x, y, z) hash to the value left at byte offset
-// kHashDeposit by the last deposit (synchronises)
-bool same_as_deposit(osc_ctx* c, int64_t n, const double* x, const double* y, const double* z);
+// Queues the content hash of the n points (x, y, z) and the copy of it and
+// of the last deposit's hash (byte offset kHashDeposit) to c->hash_host;
+// deposit_hash_matches waits for that copy only (work queued after the check
+// keeps the GPU busy meanwhile) and compares
+void queue_deposit_hash_check(osc_ctx* c, int64_t n, const double* x, const double* y,
+                              const double* z);
+bool deposit_hash_matches(osc_ctx* c);
 void bbox(osc_ctx* c, int64_t n, const double* x, const double* y, const double* z, double lo[3],
           double hi[3]);
 double device_sum(osc_ctx* c, const double* v, int64_t n);
@@ -375,9 +386,13 @@ struct InterpArgs {
 int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
                     int64_t n, const double* x, const double* y, const double* z, double* E,
                     double* B, int8_t* clamped);
-// same, for the particles of the preceding deposit in their cell-sorted order
+// same, for the particles of the preceding deposit in their cell-sorted order;
+// packed: the node records of (ef, bf) already queued by pack_nodes, or null
 void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
-                        const SortedParticles& sp, double* E, double* B, int8_t* clamped);
+                        const SortedParticles& sp, double* E, double* B, int8_t* clamped,
+                        const double* packed = nullptr);
+// queues the 64-byte node records (E, B) of the mesh into rec (8 doubles per node)
+void pack_nodes(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf, double* rec);
 // stable sort of n points by the mesh cell their gather reads (the clamped
 // cell of interpolate_field), for a cache-friendly gather
 void sort_for_gather(osc_ctx* c, const InterpArgs& a, int64_t n, const double* x, const double* y,
