@@ -220,6 +220,7 @@ __device__ __forceinline__ double wsum8_lds(const double* st, int q, int c, doub
 // record, about as costly as the random read it replaces in the return to
 // particle order, which then streams (k_unpack).
 constexpr int kRecPad = 9;
+constexpr int kUnpackOccLds = 17 * 1024;
 static_assert(4 * kNodeWin * 6 <= 64 * kRecPad, "node rows exceed the wave's stage");
 __global__ void __launch_bounds__(kThreads)
     k_interp_cells(const double* __restrict__ R, int wantE, int wantB, int64_t n,
@@ -408,7 +409,10 @@ void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const
       OSC_HIP(hipGetLastError());
     }
     ProfScope po(c, ST_GATHER_OUT);
-    hipLaunchKernelGGL(k_unpack, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
+    // kUnpackOccLds bytes of dynamic LDS, unused by the kernel, cap it at 4
+    // workgroups (4 waves per SIMD) per CU: measured 223 us against 233 us at
+    // the 8 waves per SIMD its registers allow (10^7 records; 3 waves: 239 us)
+    hipLaunchKernelGGL(k_unpack, dim3(unsigned(nb)), dim3(kThreads), kUnpackOccLds, c->stream,
                        out8.as<const double>(), n, (E && ef) ? E : nullptr, (B && bf) ? B : nullptr,
                        clamped);
     OSC_HIP(hipGetLastError());
