@@ -29,7 +29,7 @@ void axis_t(int dir, const cpx<T>* in, cpx<T>* out, const AxisIO& a, int g1, int
 template <int L>
 void xr2c_t(const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw, hipStream_t s,
             const CornerDiff* cd) {
-  constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
+  constexpr int BX = XGeo<T, L>::BX, NT = XGeo<T, L>::NT;
   dim3 g = grid3(cdiv(cdiv(a.nrows, 2), BX), 1, 1);
   if (cd)
     hipLaunchKernelGGL((k_xr2c<T, L, true>), g, dim3(NT), 0, s, in, out, a, tw, *cd);
@@ -38,7 +38,7 @@ void xr2c_t(const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw, hip
 }
 template <int L>
 void xc2r_t(const cpx<T>* in, const C2RIO<T>& a, int nout, const cpx<T>* tw, hipStream_t s) {
-  constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
+  constexpr int BX = XGeo<T, L>::BX, NT = XGeo<T, L>::NT;
   dim3 g = grid3(cdiv(cdiv(a.nrows, 2), BX), nout, 1);
   hipLaunchKernelGGL((k_xc2r<T, L>), g, dim3(NT), 0, s, in, a, tw);
 }

@@ -125,6 +125,14 @@ struct Geo {
   static constexpr int NT = BX * TPL;
 };
 
+// x-pass geometry (contiguous rows, so no cache-line constraint on the
+// number of lines): half the lines of Geo per workgroup, half its LDS
+template <typename T, int L>
+struct XGeo {
+  static constexpr int BX = Geo<T, L>::BX >= 2 ? Geo<T, L>::BX / 2 : 1;
+  static constexpr int NT = BX * Geo<T, L>::TPL;
+};
+
 // generic engine geometry: one line per 64-thread workgroup
 constexpr int kGenTPL = 64;
 constexpr int kGenBX = 1;
@@ -248,10 +256,10 @@ __device__ __forceinline__ double diff_elem(const double* q, const R2CIO& a, con
 }
 
 template <typename T, int L, bool DIFF>
-__global__ void __launch_bounds__((Geo<T, L>::NT))
+__global__ void __launch_bounds__((XGeo<T, L>::NT))
     k_xr2c(const double* __restrict__ in, cpx<T>* __restrict__ out, R2CIO a,
            const cpx<T>* __restrict__ tw, CornerDiff cd) {
-  constexpr int BX = Geo<T, L>::BX;
+  constexpr int BX = XGeo<T, L>::BX;
   using F = FftTile<T, L, -1, BX, true>;
   __shared__ cpx<T> lds[F::LDS];
   F f;
@@ -351,9 +359,9 @@ __device__ __forceinline__ void epi_store(const Epilogue& e, int64_t off, double
 }
 
 template <typename T, int L>
-__global__ void __launch_bounds__((Geo<T, L>::NT))
+__global__ void __launch_bounds__((XGeo<T, L>::NT))
     k_xc2r(const cpx<T>* __restrict__ in, C2RIO<T> a, const cpx<T>* __restrict__ tw) {
-  constexpr int BX = Geo<T, L>::BX;
+  constexpr int BX = XGeo<T, L>::BX;
   using F = FftTile<T, L, 1, BX, true>;
   __shared__ cpx<T> lds[F::LDS];
   F f;
