@@ -153,6 +153,8 @@ struct KeyParams {
   uint32_t sentinel;
   int clamp;  // 0: deposit rule (off-mesh particles get the sentinel); 1: gather rule
   const MeshDev* md;  // if set: mn and dinv come from here (a mesh resized on the device)
+  uint32_t* clear;    // if set: nclear words zeroed by the kernel (the cell flags of the
+  int64_t nclear;     // deposit, cleared here instead of by a launch of their own)
 };
 
 __device__ __forceinline__ bool cell_of(double x, double mn, double dinv, double base, int nlo,
@@ -264,6 +266,10 @@ __global__ void __launch_bounds__(KEY_T)
       p.dinv[d] = p.md->dinv[d];
     }
   __syncthreads();
+  if (p.clear)
+    for (int64_t w = blockIdx.x * int64_t(KEY_T) + threadIdx.x; w < p.nclear;
+         w += int64_t(gridDim.x) * KEY_T)
+      p.clear[w] = 0u;
   uint32_t mine = 0;
   uint64_t hs = 0;
   const int64_t base = int64_t(blockIdx.x) * RS_TILE;
@@ -1024,6 +1030,18 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   while ((uint64_t(1) << nbits) <= uint64_t(ncell)) ++nbits;
   const SortPlan sp = sort_plan(nbits, n);
   DevBuf hist(c->pool, sizeof(uint32_t) * 256 * size_t(sp.nb + 1));
+  // one flag per cell, set where a cell's sums are written (S itself is not
+  // cleared: only flagged cells are read); zeroed by the key kernel when its
+  // grid has a thread per 4 words, else by a fill of their own
+  const int64_t occ_words = (ncell + 3) / 4;
+  DevBuf occ(c->pool, sizeof(uint32_t) * occ_words);
+  if (n > 0 && int64_t(sp.nb) * KEY_T * 4 >= occ_words) {
+    kp.clear = occ.as<uint32_t>();
+    kp.nclear = occ_words;
+  } else {
+    ProfScope pf(c, ST_DEP_FIX);
+    fill_u32(c, occ.as<uint32_t>(), occ_words, 0u);
+  }
   if (n > 0) {
     ProfScope ps(c, ST_DEP_KEYS);
     DevBuf hpart(c->pool, a.hash ? sizeof(uint64_t) * sp.nb : 0);
@@ -1049,14 +1067,6 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   const int64_t nw = (n + 63) / 64 + 1;
   DevBuf S(c->pool, sizeof(double) * 8 * ncell), head(c->pool, sizeof(double) * 8 * nw),
       tail(c->pool, sizeof(double) * 8 * nw), heavy(c->pool, sizeof(uint32_t) * nw);
-  // one flag per cell, set where a cell's sums are written (S itself is not
-  // cleared: only flagged cells are read)
-  const int64_t occ_words = (ncell + 3) / 4;
-  DevBuf occ(c->pool, sizeof(uint32_t) * occ_words);
-  {
-    ProfScope pf(c, ST_DEP_FIX);
-    fill_u32(c, occ.as<uint32_t>(), occ_words, 0u);
-  }
   if (n > 0) {
     ProfScope ps_cells(c, ST_DEP_CELLS);
     if (keep)
