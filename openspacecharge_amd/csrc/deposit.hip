@@ -143,6 +143,8 @@ __global__ void k_sum_final(const double* __restrict__ part, int nb, double* __r
 constexpr int RS_T = 512, RS_I = 10, RS_TILE = RS_T * RS_I, RS_W = RS_T / 64;
 static_assert(RS_T >= 256, "one thread per digit in the scatter prologue");
 constexpr int KEY_T = 1024, KEY_I = RS_TILE / KEY_T;
+static_assert(RS_TILE % KEY_T == 0, "the key kernel must cover whole sort tiles");
+static_assert((RS_TILE / RS_W) % 64 == 0, "each wave of the scatter ranks whole 64-pair steps");
 
 // ---------------------------------------------------------------------------
 // step 1: cell keys
