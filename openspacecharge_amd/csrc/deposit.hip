@@ -350,42 +350,6 @@ __global__ void __launch_bounds__(RS_T)
   for (int d = threadIdx.x; d <= int(mask); d += RS_T) hist[int64_t(d) * nb + blockIdx.x] = h[d];
 }
 
-// Per-block digit counts hist[d * nb + b] -> their exclusive scan over the
-// blocks of each digit (in place), and each digit's total.  One block per
-// digit; thread t scans a contiguous slice of the row.
-__global__ void __launch_bounds__(RS_T)
-    k_digit_scan(uint32_t* __restrict__ hist, int nb, uint32_t* __restrict__ total) {
-  __shared__ uint32_t s[RS_T];
-  uint32_t* row = hist + int64_t(blockIdx.x) * nb;
-  const int per = (nb + RS_T - 1) / RS_T;
-  const int b0 = int(threadIdx.x) * per;
-  uint32_t acc = 0;
-  for (int q = 0; q < per; ++q)
-    if (b0 + q < nb) acc += row[b0 + q];
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  for (int off = 1; off < RS_T; off <<= 1) {
-    const uint32_t a = int(threadIdx.x) >= off ? s[threadIdx.x - off] : 0u;
-    __syncthreads();
-    s[threadIdx.x] += a;
-    __syncthreads();
-  }
-  uint32_t run = s[threadIdx.x] - acc;
-  for (int q = 0; q < per; ++q)
-    if (b0 + q < nb) {
-      const uint32_t v = row[b0 + q];
-      row[b0 + q] = run;
-      run += v;
-    }
-  if (threadIdx.x == RS_T - 1) total[blockIdx.x] = s[RS_T - 1];
-}
-
-// payload gathered after the last pass: the particle records in sorted order
-struct SortPayload {
-  const double4* rec;
-  double4* srec;
-};
-
 // inclusive scan of one value per thread over the RS_T threads: within each
 // wave by shuffles, then the wave totals (s holds RS_W entries)
 __device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t* s) {
@@ -401,6 +365,36 @@ __device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t* s) {
   __syncthreads();
   return v;
 }
+
+// Per-block digit counts hist[d * nb + b] -> their exclusive scan over the
+// blocks of each digit (in place), and each digit's total.  One block per
+// digit; thread t scans a contiguous slice of the row.
+__global__ void __launch_bounds__(RS_T)
+    k_digit_scan(uint32_t* __restrict__ hist, int nb, uint32_t* __restrict__ total) {
+  __shared__ uint32_t s[RS_W];
+  uint32_t* row = hist + int64_t(blockIdx.x) * nb;
+  const int per = (nb + RS_T - 1) / RS_T;
+  const int b0 = int(threadIdx.x) * per;
+  uint32_t acc = 0;
+  for (int q = 0; q < per; ++q)
+    if (b0 + q < nb) acc += row[b0 + q];
+  const uint32_t incl = block_scan_incl(acc, s);
+  uint32_t run = incl - acc;
+  for (int q = 0; q < per; ++q)
+    if (b0 + q < nb) {
+      const uint32_t v = row[b0 + q];
+      row[b0 + q] = run;
+      run += v;
+    }
+  if (threadIdx.x == RS_T - 1) total[blockIdx.x] = incl;
+}
+
+// payload gathered after the last pass: the particle records in sorted order
+struct SortPayload {
+  const double4* rec;
+  double4* srec;
+};
+
 
 // the LDS operations of one wave run in program order; the fences keep the
 // compiler from moving accesses across (other lanes read what was written)
