@@ -13,6 +13,10 @@
  * library and FFT-based fields agree to round-off.  The FFT is this file's own
  * mixed-radix transform (any exact DFT agrees with the upstream Singleton FFT
  * to round-off; only its sign convention and normalisation are restated).
+ * FFT lines, Green-function points, spectral products and interpolation
+ * points are independent and run on OpenMP threads; each is computed the same
+ * way on any thread, so results do not depend on the thread count.  The
+ * deposit accumulates in particle order on one thread, as upstream.
  *
  * Parity status of every function is pinned by tests/test_oracle_*.py
  * (against the compiled upstream library, numpy.fft, brute force, quadrature
@@ -109,22 +113,28 @@ static int fft_axis(cplx *a, int n1, int n2, int n3, int axis, int sign) {
   int fac[40];
   if (n == 1) return 0;
   if (factorize(n, fac) < 0) return -1;
-  cplx *W = malloc(sizeof(cplx) * n), *lin = malloc(sizeof(cplx) * n), *lout = malloc(sizeof(cplx) * n);
+  cplx *W = malloc(sizeof(cplx) * n);
   twiddles(W, n, sign);
   int64_t stride = axis == 0 ? 1 : axis == 1 ? n1 : (int64_t)n1 * n2;
   int64_t nlines = (int64_t)n1 * n2 * n3 / n;
-  for (int64_t l = 0; l < nlines; ++l) {
-    int64_t base;
-    if (axis == 0) base = l * n1;
-    else if (axis == 1) base = (l % n1) + (l / n1) * (int64_t)n1 * n2;
-    else base = l;
-    for (int j = 0; j < n; ++j) lin[j] = a[base + j * stride];
-    fft_rec(lin, 1, lout, n, fac, W, n);
-    for (int j = 0; j < n; ++j) a[base + j * stride] = lout[j];
+  /* lines are independent: each is transformed the same way on any thread */
+#pragma omp parallel if (nlines * n >= 65536)
+  {
+    cplx *lin = malloc(sizeof(cplx) * n), *lout = malloc(sizeof(cplx) * n);
+#pragma omp for schedule(static)
+    for (int64_t l = 0; l < nlines; ++l) {
+      int64_t base;
+      if (axis == 0) base = l * n1;
+      else if (axis == 1) base = (l % n1) + (l / n1) * (int64_t)n1 * n2;
+      else base = l;
+      for (int j = 0; j < n; ++j) lin[j] = a[base + j * stride];
+      fft_rec(lin, 1, lout, n, fac, W, n);
+      for (int j = 0; j < n; ++j) a[base + j * stride] = lout[j];
+    }
+    free(lin);
+    free(lout);
   }
   free(W);
-  free(lin);
-  free(lout);
   return 0;
 }
 
@@ -300,14 +310,29 @@ int64_t oc_depose_rho_scalar(int64_t n, const double *x, const double *y, const 
 
 /* Conditioning probe: when nonzero, every IGF primitive value is perturbed by
    a pseudo-random -1/0/+1 ulp.  The spread this causes in a field is the
-   floor below which no FP64 evaluation of the upstream formulas can agree. */
+   floor below which no FP64 evaluation of the upstream formulas can agree.
+   The choice of -1/0/+1 is a hash of the seed, the primitive's arguments and
+   its value (splitmix64 finaliser), so it does not depend on evaluation order
+   and the loops that call the primitives may run on several threads. */
 static uint64_t g_perturb = 0;
 void oc_set_perturb(uint64_t seed) { g_perturb = seed; }
 
-static double perturb(double v) {
+static uint64_t mix64(uint64_t h) {
+  h ^= h >> 30;
+  h *= 0xbf58476d1ce4e5b9ULL;
+  h ^= h >> 27;
+  h *= 0x94d049bb133111ebULL;
+  return h ^ (h >> 31);
+}
+static uint64_t dbits(double d) {
+  uint64_t u;
+  memcpy(&u, &d, sizeof(u));
+  return u;
+}
+static double perturb3(double v, double x, double y, double z) {
   if (!g_perturb) return v;
-  g_perturb = g_perturb * 6364136223846793005ULL + 1442695040888963407ULL;
-  int u = (int)((g_perturb >> 33) % 3) - 1;
+  uint64_t h = mix64(g_perturb ^ mix64(dbits(x) ^ mix64(dbits(y) ^ mix64(dbits(z) ^ mix64(dbits(v))))));
+  int u = (int)((h >> 33) % 3) - 1;
   return u == 0 ? v : nextafter(v, u > 0 ? INFINITY : -INFINITY);
 }
 
@@ -325,23 +350,26 @@ void oc_set_zero_unused_planes(int on) { g_zero_raw = on; }
 /* xlafun2 (open_spacecharge_mod.f90:721-726) */
 double oc_xlafun2(double x, double y, double z) {
   double r = sqrt(x * x + y * y + z * z);
-  return perturb(x * atan((y * z) / (r * x)) - z * log(r + y) + y * log((r - z) / (r + z)) / 2.0);
+  return perturb3(x * atan((y * z) / (r * x)) - z * log(r + y) + y * log((r - z) / (r + z)) / 2.0, x, y,
+                  z);
 }
 
 /* lafun2 (open_spacecharge_mod.f90:741-747) */
 double oc_lafun2(double x, double y, double z) {
   double r = sqrt(x * x + y * y + z * z);
-  return perturb(-(z * z * atan(x * y / (z * r)) / 2.0) - y * y * atan(x * z / (y * r)) / 2.0 -
-                 x * x * atan(y * z / (x * r)) / 2.0 + y * z * log(x + r) + x * z * log(y + r) +
-                 x * y * log(z + r));
+  return perturb3(-(z * z * atan(x * y / (z * r)) / 2.0) - y * y * atan(x * z / (y * r)) / 2.0 -
+                      x * x * atan(y * z / (x * r)) / 2.0 + y * z * log(x + r) + x * z * log(y + r) +
+                      x * y * log(z + r),
+                  x, y, z);
 }
 
 /* lafun (core:229-238) */
 double oc_lafun(double x, double y, double z) {
   double r = sqrt(x * x + y * y + z * z);
-  return perturb(-(0.5 * (z * z) * atan(x * y / (z * r))) - 0.5 * (y * y) * atan(x * z / (y * r)) -
-                 0.5 * (x * x) * atan(y * z / (x * r)) + y * z * log(x + r) + x * z * log(y + r) +
-                 x * y * log(z + r));
+  return perturb3(-(0.5 * (z * z) * atan(x * y / (z * r))) - 0.5 * (y * y) * atan(x * z / (y * r)) -
+                      0.5 * (x * x) * atan(y * z / (x * r)) + y * z * log(x + r) + x * z * log(y + r) +
+                      x * y * log(z + r),
+                  x, y, z);
 }
 
 /* xlafun (core:377-392) */
@@ -350,7 +378,7 @@ double oc_xlafun(double x, double y, double z) {
   double res = z - x * atan(z / x) + x * atan(y * z / (x * r));
   if (y + r != 0) res = res - z * log(y + r);
   if (z + r != 0) res = res - y * log(z + r);
-  return perturb(res);
+  return perturb3(res, x, y, z);
 }
 
 /* ylafun (core:397-407) */
@@ -359,7 +387,7 @@ double oc_ylafun(double x, double y, double z) {
   double res = x - y * atan(x / y) + y * atan(z * x / (y * r));
   if (z + r != 0) res = res - x * log(z + r);
   if (x + r != 0) res = res - z * log(x + r);
-  return perturb(res);
+  return perturb3(res, x, y, z);
 }
 
 /* zlafun (core:412-428) */
@@ -368,13 +396,13 @@ double oc_zlafun(double x, double y, double z) {
   double res = y - z * atan(y / z) + z * atan(x * y / (z * r));
   if (x + r != 0) res = res - y * log(x + r);
   if (y + r != 0) res = res - x * log(y + r);
-  return perturb(res);
+  return perturb3(res, x, y, z);
 }
 
 /* coulombfun (core:189-200) */
 static double coulombfun(double u, double v, double w, double gam) {
   if (u == 0.0 && v == 0.0 && w == 0.0) return 0.0;
-  return perturb(1.0 / sqrt(u * u + v * v + (gam * w) * (gam * w)));
+  return perturb3(1.0 / sqrt(u * u + v * v + (gam * w) * (gam * w)), u, v, w);
 }
 
 typedef double (*lafun_t)(double, double, double);
@@ -425,7 +453,7 @@ double oc_rfun(double u, double v, double w, double gam, double a, double b, dou
       res = res + term;
     }
   res = res * 2.0 * pi * ainv * binv;
-  return perturb(res);
+  return perturb3(res, u, v, w);
 }
 
 /* ========================================================================== */
@@ -447,6 +475,7 @@ void oc_get_cgrn_freespace(double *cgrn, int isize, int jsize, int ksize,
     wmin = wmin + offset[2] * gamma;
   }
   cplx *g = (cplx *)cgrn;
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < ksize; ++k) {
     double w = k * dz + wmin;
     for (int j = 0; j < jsize; ++j) {
@@ -467,6 +496,7 @@ void oc_get_cgrn_freespace(double *cgrn, int isize, int jsize, int ksize,
   double *t = malloc(sizeof(double) * nt);
   for (int64_t i = 0; i < nt; ++i) t[i] = g[i].re;
 #define T(i, j, k) t[IDX3(i, j, k, isize, jsize)]
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < ksize - 1; ++k)
     for (int j = 0; j < jsize - 1; ++j)
       for (int i = 0; i < isize - 1; ++i) {
@@ -494,6 +524,7 @@ static void grn_legacy(double *cgrn, const int lo[3], const int per[3], const in
   int jshift = per[1] / 2 - (npad[1] + 1) / 2;
   int kshift = per[2] / 2 - (npad[2] + 1) / 2;
   cplx *g = (cplx *)cgrn;
+#pragma omp parallel for schedule(static)
   for (int kk = 0; kk < per[2]; ++kk) {
     int k = lo[2] + kk;
     double w;
@@ -573,6 +604,7 @@ void oc_getgrnpipe(double *g[4], const int nlo[3], const int nhi[3], const int n
   const int sx[4] = {1, 1, -1, -1}, sy[4] = {1, -1, 1, -1};
   for (int t = 0; t < 4; ++t) {
     cplx *c = (cplx *)g[t];
+#pragma omp parallel for schedule(static)
     for (int kk = 0; kk < per[2]; ++kk) {
       int kp = g1lo[2] + (kk + kshift) % per[2];
       double w = kp * hz;
@@ -623,6 +655,7 @@ void oc_freespace_solver2(int nx, int ny, int nz, const double *rho, double gamm
     if (icomp == 1 && !efield) break;
     oc_get_cgrn_freespace((double *)cgrn, nx2, ny2, nz2, delta, gamma, icomp, offset);
     oc_ccfft3d((double *)cgrn, nx2, ny2, nz2, fwd);
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N2; ++i) cgrn[i] = cmul(crho[i], cgrn[i]);
     oc_ccfft3d((double *)cgrn, nx2, ny2, nz2, inv);
     int ish = nx - 1, jsh = ny - 1, ksh = nz - 1;
@@ -690,6 +723,7 @@ static cplx *getrhotilde(const double *rho, int nx, int ny, int nz, const int pe
 static cplx *convterm(const cplx *crho, const cplx *g, const int per[3], const int dir[3]) {
   int64_t N = (int64_t)per[0] * per[1] * per[2];
   cplx *c = malloc(N * sizeof(cplx));
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < N; ++i) c[i] = cmul(crho[i], g[i]);
   oc_ccfft3d((double *)c, per[0], per[1], per[2], dir);
   return c;
@@ -914,6 +948,7 @@ void oc_interpolate_field(const int nlo[3], const int nhi[3], const double mn[3]
   int nx = nhi[0] - nlo[0] + 1, ny = nhi[1] - nlo[1] + 1, nz = nhi[2] - nlo[2] + 1;
   int64_t nt = (int64_t)nx * ny * nz;
   double hxi = 1.0 / delta[0], hyi = 1.0 / delta[1], hzi = 1.0 / delta[2];
+#pragma omp parallel for schedule(static)
   for (int64_t p = 0; p < n; ++p) {
     int ip = (int)floor((x[p] - mn[0]) * hxi + 1.0);
     int jp = (int)floor((y[p] - mn[1]) * hyi + 1.0);
