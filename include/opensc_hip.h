@@ -95,6 +95,9 @@ int osc_ctx_profile_read(osc_ctx* ctx, double* ms, int64_t* launches);
 const char* osc_stage_name(int stage);
 /* 1 if this transform length is supported (prime factors <= 23 and fits LDS) */
 int osc_fft_length_supported(int n);
+/* 1 if this length has a compile-time radix plan (LDS-tiled multi-line passes);
+ * other supported lengths use the runtime-radix engine, one line per workgroup */
+int osc_fft_length_planned(int n);
 
 /* ---- deposition --------------------------------------------------------------
  * deposit_particles (open_spacecharge_mod.f90:156-288).

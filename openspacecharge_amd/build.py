@@ -87,7 +87,8 @@ def build_fortran(quiet: bool = True, driver: bool = True) -> str | None:
     objs = []
     for src in F_SOURCES:
         obj = os.path.join(BUILD, src.replace(".f90", "_f.o"))
-        _run([FLANG, "-O2", "-fPIC", "-cpp", "-ffree-form", "-module-dir", F_MODDIR, "-c",
+        _run([FLANG, "-O2", "-ffp-contract=off", "-fPIC", "-cpp", "-ffree-form", "-module-dir",
+              F_MODDIR, "-c",
               os.path.join(FSRC, src), "-o", obj], quiet)
         objs.append(obj)
     tmp = F_LIB + ".tmp"

@@ -65,6 +65,8 @@ OSC_API int osc_abi_version(void) { return OSC_ABI_VERSION; }
 
 OSC_API int osc_fft_length_supported(int n) { return fft_len_supported(n) ? 1 : 0; }
 
+OSC_API int osc_fft_length_planned(int n) { return fft_has_plan(n) ? 1 : 0; }
+
 OSC_API int osc_ctx_create(int device, osc_ctx** out) {
   if (!out) return OSC_ERR_ARG;
   *out = nullptr;

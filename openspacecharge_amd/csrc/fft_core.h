@@ -131,43 +131,46 @@ constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
 template <int L>
 struct FftPlan {
   static constexpr int ns = 0;  // no specialised plan
-  static constexpr int r[4] = {1, 1, 1, 1};
+  static constexpr int r[5] = {1, 1, 1, 1, 1};
 };
 
-#define OSC_PLAN(LL, NS, R0, R1, R2, R3)            \
-  template <>                                       \
-  struct FftPlan<LL> {                              \
-    static constexpr int ns = NS;                   \
-    static constexpr int r[4] = {R0, R1, R2, R3};   \
+#define OSC_PLAN(LL, NS, R0, R1, R2, R3, R4)             \
+  template <>                                            \
+  struct FftPlan<LL> {                                   \
+    static constexpr int ns = NS;                        \
+    static constexpr int r[5] = {R0, R1, R2, R3, R4};    \
   };
 
 // L = 2n for the doubled grids of n = 2^a 3^b (16 <= L <= 1024)
-OSC_PLAN(16, 2, 4, 4, 1, 1)
-OSC_PLAN(24, 2, 6, 4, 1, 1)
-OSC_PLAN(32, 2, 8, 4, 1, 1)
-OSC_PLAN(36, 2, 6, 6, 1, 1)
-OSC_PLAN(48, 3, 4, 4, 3, 1)
-OSC_PLAN(54, 3, 6, 3, 3, 1)
-OSC_PLAN(64, 2, 8, 8, 1, 1)
-OSC_PLAN(72, 3, 6, 6, 2, 1)
-OSC_PLAN(96, 3, 6, 4, 4, 1)
-OSC_PLAN(108, 3, 6, 6, 3, 1)
-OSC_PLAN(128, 3, 8, 4, 4, 1)
-OSC_PLAN(144, 3, 6, 6, 4, 1)
-OSC_PLAN(162, 4, 6, 3, 3, 3)
-OSC_PLAN(192, 4, 4, 4, 4, 3)
-OSC_PLAN(216, 3, 6, 6, 6, 1)
-OSC_PLAN(256, 3, 8, 8, 4, 1)
-OSC_PLAN(288, 4, 4, 4, 6, 3)
-OSC_PLAN(324, 4, 6, 6, 3, 3)
-OSC_PLAN(384, 4, 4, 4, 4, 6)
-OSC_PLAN(432, 4, 6, 6, 6, 2)
-OSC_PLAN(512, 3, 8, 8, 8, 1)
-OSC_PLAN(576, 4, 4, 4, 6, 6)
-OSC_PLAN(648, 4, 6, 6, 6, 3)
-OSC_PLAN(768, 4, 8, 4, 4, 6)
-OSC_PLAN(864, 4, 6, 6, 6, 4)
-OSC_PLAN(1024, 4, 8, 8, 4, 4)
+OSC_PLAN(16, 2, 4, 4, 1, 1, 1)
+OSC_PLAN(18, 2, 6, 3, 1, 1, 1)
+OSC_PLAN(24, 2, 6, 4, 1, 1, 1)
+OSC_PLAN(32, 2, 8, 4, 1, 1, 1)
+OSC_PLAN(36, 2, 6, 6, 1, 1, 1)
+OSC_PLAN(48, 3, 4, 4, 3, 1, 1)
+OSC_PLAN(54, 3, 6, 3, 3, 1, 1)
+OSC_PLAN(64, 2, 8, 8, 1, 1, 1)
+OSC_PLAN(72, 3, 6, 6, 2, 1, 1)
+OSC_PLAN(96, 3, 6, 4, 4, 1, 1)
+OSC_PLAN(108, 3, 6, 6, 3, 1, 1)
+OSC_PLAN(128, 3, 8, 4, 4, 1, 1)
+OSC_PLAN(144, 3, 6, 6, 4, 1, 1)
+OSC_PLAN(162, 4, 6, 3, 3, 3, 1)
+OSC_PLAN(192, 4, 4, 4, 4, 3, 1)
+OSC_PLAN(216, 3, 6, 6, 6, 1, 1)
+OSC_PLAN(256, 3, 8, 8, 4, 1, 1)
+OSC_PLAN(288, 4, 4, 4, 6, 3, 1)
+OSC_PLAN(324, 4, 6, 6, 3, 3, 1)
+OSC_PLAN(384, 4, 4, 4, 4, 6, 1)
+OSC_PLAN(432, 4, 6, 6, 6, 2, 1)
+OSC_PLAN(486, 5, 6, 3, 3, 3, 3)
+OSC_PLAN(512, 3, 8, 8, 8, 1, 1)
+OSC_PLAN(576, 4, 4, 4, 6, 6, 1)
+OSC_PLAN(648, 4, 6, 6, 6, 3, 1)
+OSC_PLAN(768, 4, 8, 4, 4, 6, 1)
+OSC_PLAN(864, 4, 6, 6, 6, 4, 1)
+OSC_PLAN(972, 5, 6, 6, 3, 3, 3)
+OSC_PLAN(1024, 4, 8, 8, 4, 4, 1)
 #undef OSC_PLAN
 
 template <int L>

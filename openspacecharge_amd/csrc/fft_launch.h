@@ -7,8 +7,9 @@ namespace osc {
 
 // lengths with a compile-time plan
 #define OSC_FOR_EACH_L(X)                                                                   \
-  X(16) X(24) X(32) X(36) X(48) X(54) X(64) X(72) X(96) X(108) X(128) X(144) X(162) X(192) \
-      X(216) X(256) X(288) X(324) X(384) X(432) X(512) X(576) X(648) X(768) X(864) X(1024)
+  X(16) X(18) X(24) X(32) X(36) X(48) X(54) X(64) X(72) X(96) X(108) X(128) X(144) X(162)   \
+      X(192) X(216) X(256) X(288) X(324) X(384) X(432) X(486) X(512) X(576) X(648) X(768)  \
+      X(864) X(972) X(1024)
 
 bool fft_has_plan(int L);
 // true when every prime factor of L is <= 23 and the generic engine fits LDS

@@ -242,46 +242,74 @@ subroutine deposit_particles(xa, ya, za, mesh3d, qa, total_charge, resize_mesh)
 end subroutine deposit_particles
 
 !------------------------------------------------------------------------
-! interpolate_field (open_spacecharge_mod.f90:311-390), one point.  Each
-! call is one device round trip; interpolate_field_batch does many points.
+! interpolate_field (open_spacecharge_mod.f90:311-390), one point, on the
+! host.  Callers interpolate one point per call inside their particle loops
+! (code/test_opensc.f90:237-295) and mesh3d%efield/bfield are host arrays,
+! so the 8-node sum is taken here, in upstream's operation order (index and
+! weight arithmetic, clamping, left-to-right products and sum; this file is
+! compiled without FMA contraction), which is also the order of the device
+! gather: the result is bit-identical to interpolate_field_batch.  Upstream
+! prints ab, de, gh after the j message before computing them (values left
+! from the previous call); the same values are printed here.
 !------------------------------------------------------------------------
 subroutine interpolate_field(x, y, z, mesh3d, E, B)
-  type(mesh3d_struct), target :: mesh3d
-  real(dp), target :: x, y, z
-  real(dp), optional, target :: E(3), B(3)
-  type(osc_mesh_c) :: m
-  type(c_ptr) :: pe, pb, fe, fb
-  integer(c_int) :: rc
-  integer(c_int64_t) :: ncl
-  integer :: ip, jp, kp
+  type(mesh3d_struct) :: mesh3d
+  real(dp) :: x, y, z
+  real(dp), optional :: E(3), B(3)
+  real(dp), save :: ab = 0, de = 0, gh = 0
+  real(dp) :: hxi, hyi, hzi
+  integer :: ip, jp, kp, ip1, jp1, kp1
+  logical :: clamped
 
-  m = to_c(mesh3d)
-  pe = c_null_ptr; pb = c_null_ptr; fe = c_null_ptr; fb = c_null_ptr
+  clamped = .false.
+  hxi = 1.d0 / mesh3d%delta(1); hyi = 1.d0 / mesh3d%delta(2); hzi = 1.d0 / mesh3d%delta(3)
+  ip = floor((x - mesh3d%min(1)) * hxi + 1)
+  jp = floor((y - mesh3d%min(2)) * hyi + 1)
+  kp = floor((z - mesh3d%min(3)) * hzi + 1)
+  if (ip < 1 .or. ip > mesh3d%nhi(1) - 1) then
+    clamped = .true.
+    write(6, *) 'ierror: ip=', ip, (x - mesh3d%min(1)), (x - mesh3d%min(1)) / mesh3d%delta(1)
+    ip = merge(1, mesh3d%nhi(1) - 1, ip < 1)
+  endif
+  if (jp < 1 .or. jp > mesh3d%nhi(2) - 1) then
+    clamped = .true.
+    write(6, *) 'jerror: jp=', jp, (y - mesh3d%min(2)), (y - mesh3d%min(2)) / mesh3d%delta(2)
+    write(6, *) ab, de, gh
+    jp = merge(1, mesh3d%nhi(2) - 1, jp < 1)
+  endif
+  if (kp < 1 .or. kp > mesh3d%nhi(3) - 1) then
+    clamped = .true.
+    write(6, *) 'kerror:  kp=', kp
+    write(6, *) 'z=', z
+    write(6, *) 'mesh3d%min(3)=', mesh3d%min(3)
+    kp = merge(1, mesh3d%nhi(3) - 1, kp < 1)
+  endif
+  ab = ((mesh3d%min(1) - x) + ip * mesh3d%delta(1)) * hxi
+  de = ((mesh3d%min(2) - y) + jp * mesh3d%delta(2)) * hyi
+  gh = ((mesh3d%min(3) - z) + kp * mesh3d%delta(3)) * hzi
+  if (clamped) write(6, *) ab, de, gh
+  ip1 = ip + 1
+  jp1 = jp + 1
+  kp1 = kp + 1
   if (present(E)) then
-    pe = c_loc(E)
-    fe = c_loc(mesh3d%efield)
+    E = mesh3d%efield(ip, jp,  kp,  :)*ab*de*gh                 &
+      + mesh3d%efield(ip, jp1, kp,  :)*ab*(1.-de)*gh            &
+      + mesh3d%efield(ip, jp1, kp1, :)*ab*(1.-de)*(1.-gh)       &
+      + mesh3d%efield(ip, jp,  kp1, :)*ab*de*(1.-gh)            &
+      + mesh3d%efield(ip1,jp,  kp1, :)*(1.-ab)*de*(1.-gh)       &
+      + mesh3d%efield(ip1,jp1, kp1, :)*(1.-ab)*(1.-de)*(1.-gh)  &
+      + mesh3d%efield(ip1,jp1, kp,  :)*(1.-ab)*(1.-de)*gh       &
+      + mesh3d%efield(ip1,jp,  kp,  :)*(1.-ab)*de*gh
   endif
   if (present(B)) then
-    pb = c_loc(B)
-    fb = c_loc(mesh3d%bfield)
-  endif
-  ncl = 0
-  rc = c_osc_interpolate_field(ctx(), m, fe, fb, 1_c_int64_t, c_loc(x), c_loc(y), c_loc(z), pe, pb, &
-                               c_null_ptr, ncl)
-  if (rc /= OSC_OK) call fail(rc, 'interpolate_field')
-  if (ncl > 0) then   ! the upstream clamp messages (:324-362)
-    ip = floor((x - mesh3d%min(1)) / mesh3d%delta(1) + 1)
-    jp = floor((y - mesh3d%min(2)) / mesh3d%delta(2) + 1)
-    kp = floor((z - mesh3d%min(3)) / mesh3d%delta(3) + 1)
-    if (ip < 1 .or. ip > mesh3d%nhi(1) - 1) &
-      write(6, *) 'ierror: ip=', ip, (x - mesh3d%min(1)), (x - mesh3d%min(1)) / mesh3d%delta(1)
-    if (jp < 1 .or. jp > mesh3d%nhi(2) - 1) &
-      write(6, *) 'jerror: jp=', jp, (y - mesh3d%min(2)), (y - mesh3d%min(2)) / mesh3d%delta(2)
-    if (kp < 1 .or. kp > mesh3d%nhi(3) - 1) then
-      write(6, *) 'kerror:  kp=', kp
-      write(6, *) 'z=', z
-      write(6, *) 'mesh3d%min(3)=', mesh3d%min(3)
-    endif
+    B = mesh3d%bfield(ip, jp,  kp,  :)*ab*de*gh                 &
+      + mesh3d%bfield(ip, jp1, kp,  :)*ab*(1.-de)*gh            &
+      + mesh3d%bfield(ip, jp1, kp1, :)*ab*(1.-de)*(1.-gh)       &
+      + mesh3d%bfield(ip, jp,  kp1, :)*ab*de*(1.-gh)            &
+      + mesh3d%bfield(ip1,jp,  kp1, :)*(1.-ab)*de*(1.-gh)       &
+      + mesh3d%bfield(ip1,jp1, kp1, :)*(1.-ab)*(1.-de)*(1.-gh)  &
+      + mesh3d%bfield(ip1,jp1, kp,  :)*(1.-ab)*(1.-de)*gh       &
+      + mesh3d%bfield(ip1,jp,  kp,  :)*(1.-ab)*de*gh
   endif
 end subroutine
 

@@ -10,6 +10,9 @@
 !   OUT: stream file of float64 blocks in this order:
 !        min(3) max(3) delta(3) charge, rho, efield, bfield, E(3,n), B(3,n),
 !        E of the first 4 points by per-point calls (3 x 4),
+!        E(3,m), B(3,m) of the first m = min(n, 10000) points by per-point
+!        calls, the seconds those m calls took (1), E and B (3 + 3) of one
+!        point past the upper x edge (clamped; upstream prints messages),
 !        legacy direct efield, pipe phi, pipe efield
 program drive_shim
 use, intrinsic :: iso_fortran_env
@@ -19,9 +22,9 @@ integer, parameter :: dp = REAL64
 type(mesh3d_struct) :: mesh, pmesh
 character(len=512) :: fin, fout, arg
 integer :: nx, ny, nz, u, i
-integer(int64) :: n, ncl
-real(dp), allocatable :: x(:), y(:), z(:), E(:,:), B(:,:)
-real(dp) :: e1(3), apipe, bpipe, delta(3), umin(3), umax(3)
+integer(int64) :: n, ncl, m1, t0, t1, rate
+real(dp), allocatable :: x(:), y(:), z(:), E(:,:), B(:,:), Ep(:,:), Bp(:,:)
+real(dp) :: e1(3), b1(3), apipe, bpipe, delta(3), umin(3), umax(3), xo
 
 call get_command_argument(1, fin)
 call get_command_argument(2, fout)
@@ -52,6 +55,19 @@ do i = 1, 4
   call interpolate_field(x(i), y(i), z(i), mesh, E=e1)
   write(u) e1
 enddo
+! many per-point calls, as upstream callers make them in their particle loops
+m1 = min(n, 10000_int64)
+allocate(Ep(3, m1), Bp(3, m1))
+call system_clock(t0, rate)
+do i = 1, int(m1)
+  call interpolate_field(x(i), y(i), z(i), mesh, E=Ep(:, i), B=Bp(:, i))
+enddo
+call system_clock(t1)
+write(u) Ep, Bp, real(t1 - t0, dp) / real(rate, dp)
+! one point past the upper x edge: clamped to the edge cell
+xo = mesh%max(1) + 2 * mesh%delta(1)
+call interpolate_field(xo, y(1), z(1), mesh, E=e1, B=b1)
+write(u) e1, b1
 
 ! legacy free-space solver, direct field
 call space_charge_freespace(mesh, direct_field_calc=.true., integrated_green_function=.true.)

@@ -153,6 +153,20 @@ def test_fft_length_rule(lib):
     assert not lib.osc_fft_length_supported(29) and not lib.osc_fft_length_supported(2 * 31)
 
 
+def test_every_doubled_2a3b_length_is_planned(lib):
+    """Each doubled length 2n of a mesh edge n = 2^a 3^b in [8, 512] has a
+    compile-time plan (the runtime-radix engine is an order of magnitude
+    slower); the GPU plan tests (tests/test_gpu_plans.py) check exactly this
+    list, so a plan added without a test fails here."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import test_gpu_plans
+    want = sorted(2 * 2 ** a * 3 ** b for a in range(10) for b in range(7) if 8 <= 2 ** a * 3 ** b <= 512)
+    planned = [n for n in range(1, 2049) if lib.osc_fft_length_planned(n)]
+    assert planned == want == test_gpu_plans.PLANNED
+    for n in test_gpu_plans.GENERIC:
+        assert lib.osc_fft_length_supported(n) and not lib.osc_fft_length_planned(n)
+
+
 def test_stage_names(lib):
     names = [lib.osc_stage_name(i).decode() for i in range(_capi.Context.NSTAGE)]
     assert len(set(names)) == len(names) and all(names)

@@ -29,10 +29,9 @@ def _read(path, counts):
 
 
 @pytest.mark.skipif(not os.path.exists(F_DRIVER), reason="Fortran drop-in not built (amdflang)")
-@pytest.mark.parametrize("shape", [(16, 12, 20), (8, 8, 8)])
-def test_fortran_dropin_matches_c_abi(ctx, tmp_path, shape):
+@pytest.mark.parametrize("shape,n", [((16, 12, 20), 5000), ((8, 8, 8), 5000), ((64, 64, 64), 20000)])
+def test_fortran_dropin_matches_c_abi(ctx, oracle, tmp_path, shape, n):
     nx, ny, nz = shape
-    n = 5000
     x, y, z = gaussian_bunch(n, (1e-3, 0.6e-3, 2e-3), 31)
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(fin, "wb") as f:
@@ -47,8 +46,10 @@ def test_fortran_dropin_matches_c_abi(ctx, tmp_path, shape):
     assert "Error, must specify qa or total_charge" in r.stdout
     assert "drive_shim ok" in r.stdout
     ng = nx * ny * nz
-    (mn, mx, dl, charge, rho, ef, bf, E, B, e4, ef_leg, pphi, pef) = _read(
-        fout, [3, 3, 3, 1, ng, 3 * ng, 3 * ng, 3 * n, 3 * n, 12, 3 * ng, ng, 3 * ng])
+    m1 = min(n, 10000)
+    (mn, mx, dl, charge, rho, ef, bf, E, B, e4, Em, Bm, secs, eo, bo, ef_leg, pphi, pef) = _read(
+        fout, [3, 3, 3, 1, ng, 3 * ng, 3 * ng, 3 * n, 3 * n, 12, 3 * m1, 3 * m1, 1, 3, 3, 3 * ng,
+               ng, 3 * ng])
     F = lambda a, s: a.reshape(s, order="F")
 
     m = osc.Mesh3D(nhi=shape, gamma=GAMMA)
@@ -62,6 +63,20 @@ def test_fortran_dropin_matches_c_abi(ctx, tmp_path, shape):
     Ep, Bp = osc.interpolate_field(x, y, z, m, E=True, B=True)
     assert np.array_equal(E.reshape(n, 3), Ep) and np.array_equal(B.reshape(n, 3), Bp)
     assert np.array_equal(e4.reshape(4, 3), Ep[:4])
+    # per-point calls on the host (upstream order): bit-identical to the
+    # device gather and to the oracle, in microseconds per call
+    assert np.array_equal(Em.reshape(m1, 3), Ep[:m1]) and np.array_equal(Bm.reshape(m1, 3), Bp[:m1])
+    Eo, Bo, _ = oracle.interpolate_field((1, 1, 1), shape, m.min, m.delta, m.efield, m.bfield,
+                                         x[:m1], y[:m1], z[:m1])
+    assert np.array_equal(Em.reshape(m1, 3), Eo.T) and np.array_equal(Bm.reshape(m1, 3), Bo.T)
+    print(f"{m1} per-point interpolate_field calls: {secs[0] * 1e3:.2f} ms")
+    assert secs[0] < 0.05 * m1 / 1e4
+    # a point past the upper x edge is clamped, with upstream's message
+    xo = np.array([m.max[0] + 2 * m.delta[0]])
+    Ec, Bc, cl = oracle.interpolate_field((1, 1, 1), shape, m.min, m.delta, m.efield, m.bfield, xo,
+                                          y[:1], z[:1])
+    assert cl[0] == 1 and np.array_equal(eo, Ec[:, 0]) and np.array_equal(bo, Bc[:, 0])
+    assert "ierror: ip=" in r.stdout
     osc.space_charge_freespace(m, direct_field_calc=True, integrated_green_function=True)
     assert np.array_equal(F(ef_leg, shape + (3,)), m.efield)
 

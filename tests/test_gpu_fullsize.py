@@ -5,6 +5,12 @@ upstream library and the oracle are too slow to run inside a test (except the
 * 64^3 / 1e6 particles (sigma 1,1,0.1 mm): against the oracle (C restatement,
   never-extracted raw Green planes zeroed as on the GPU) within 4 x the IGF
   conditioning floor;
+* 128^3 / 1e7 particles against the oracle: the resized mesh bit-exact, rho
+  within 1e-14, E and B within 4 x the conditioning floor (raw planes zeroed
+  as on the GPU), E and B at all 1e7 particles bit-exact against the oracle's
+  interpolate_field of the same fields, and E at 40 nodes within 1e-12 of
+  max|E| of the long-double direct-sum convolution (no FFT) of the same
+  Green function;
 * 128^3 / 1e7 particles: charge conservation, additivity of the deposit and
   linearity of the solve, FP32 convolution within 1e-5 of FP64 (max over
   components of max|E32-E64|/max|E64|);
@@ -12,7 +18,8 @@ upstream library and the oracle are too slow to run inside a test (except the
   interior field (Carlson R_D) at discretisation tolerance;
 * cathode 128x128x256 / 5e6 (sigma 1,1,0.5 mm centred 1.5 mm from the
   cathode, z <= 0 rejected): the three independent image methods agree
-  (space_charge_3d at_cathode vs space_charge_cathodeimages methods 1 and 2);
+  (space_charge_3d at_cathode vs space_charge_cathodeimages methods 1 and 2),
+  and the FP32 convolution is within 1e-5 of FP64;
 * rect pipe 64x64x128 / 2e6 (10 x 6 mm pipe): the potential vanishes on the
   conducting walls.
 """
@@ -56,6 +63,67 @@ def test_config_64cube_vs_oracle(ctx, oracle):
 @pytest.fixture(scope="module")
 def bunch128():
     return gaussian_bunch(10_000_000, (0.5e-3, 0.5e-3, 2e-3), 20261202)
+
+
+def _direct_sum_nodes(rho, g, nodes):
+    """sum_j rho_j g(i - j + n - 1) at the given nodes i, in long double: the
+    doubled-grid convolution with octant extraction of osc_freespace_solver2
+    (code/open_spacecharge_mod.f90:557-593) without an FFT."""
+    n = rho.shape
+    rl = rho.astype(np.longdouble)
+    out = []
+    for (i, j, k) in nodes:
+        sub = g[i:i + n[0], j:j + n[1], k:k + n[2]][::-1, ::-1, ::-1]
+        out.append(float((rl * sub.astype(np.longdouble)).sum()))
+    return np.array(out)
+
+
+def test_config_128cube_vs_oracle(ctx, oracle, bunch128):
+    """The benchmark solve (128^3 mesh, 1e7 particles, 256^3 doubled FFT) in
+    FP64 against the oracle, and against the exact convolution at 40 nodes."""
+    print("128^3 / 1e7 vs oracle", flush=True)
+    x, y, z = bunch128
+    m = osc.Mesh3D(nhi=(128, 128, 128), gamma=GAMMA)
+    E, B = osc.space_charge_step(x, y, z, m, total_charge=1e-9, calc_bfield=True, ctx=ctx)
+    d = oracle.deposit_particles(x, y, z, (1, 1, 1), (128, 128, 128), total_charge=1e-9)
+    assert np.array_equal(d["delta"], m.delta) and np.array_equal(d["min"], m.min)
+    assert float(np.max(np.abs(m.rho - d["rho"]))) <= 1e-14 * float(np.max(np.abs(d["rho"])))
+    print("  oracle solve", flush=True)
+    oracle.set_zero_unused_planes(True)
+    try:
+        eb, fl = oracle.floor(lambda: np.concatenate(oracle.space_charge_3d(
+            m.rho, (1, 1, 1), (128, 128, 128), m.min, m.max, m.delta, GAMMA, calc_bfield=True),
+            axis=3)[..., :5], seeds=(12345,))
+    finally:
+        oracle.set_zero_unused_planes(False)
+    tol = 4 * fl + 2e-14
+    err_e = relcomp(m.efield, eb[..., :3])
+    print(f"  E vs oracle {err_e:.3e} (floor {fl:.3e})", flush=True)
+    assert err_e < tol and relcomp(m.bfield, eb[..., 3:5], 2) < tol
+    # the gather of the GPU's own fields: bit-exact at every particle
+    print("  oracle gather", flush=True)
+    Eo, Bo, _ = oracle.interpolate_field((1, 1, 1), (128, 128, 128), m.min, m.delta, m.efield,
+                                         m.bfield, x, y, z)
+    assert np.array_equal(E, Eo.T) and np.array_equal(B, Bo.T)
+    # the exact convolution at 40 nodes: 28 random, the extrema of each component
+    print("  direct sums", flush=True)
+    rng = np.random.default_rng(40)
+    nodes = [tuple(int(v) for v in rng.integers(0, 128, 3)) for _ in range(28)]
+    for c in range(3):
+        f = m.efield[..., c]
+        nodes += [np.unravel_index(int(np.argmax(f)), f.shape),
+                  np.unravel_index(int(np.argmin(f)), f.shape)]
+    nodes += [(64, 64, 64), (0, 0, 0), (127, 127, 127), (0, 127, 64), (127, 0, 1), (63, 64, 127)]
+    idx = tuple(np.array(nodes).T)
+    worst = 0.0
+    for c in range(3):
+        g = oracle.get_cgrn_freespace((256, 256, 256), m.delta, GAMMA, c + 1).real
+        exact = _direct_sum_nodes(m.rho, g, nodes) * FPEI_SOLVER2
+        del g
+        e = m.efield[..., c][idx]
+        worst = max(worst, float(np.max(np.abs(e - exact))) / float(np.max(np.abs(m.efield[..., c]))))
+    print(f"  E vs exact convolution at {len(nodes)} nodes: {worst:.3e} of max|E|", flush=True)
+    assert worst < 1e-12
 
 
 def test_config_128cube_conservation_linearity(ctx, bunch128):
@@ -147,6 +215,14 @@ def test_config_cathode_methods_agree_fullsize(ctx):
     # straddles the origin (set to 0 by the legacy Green function)
     assert relcomp(e3[:, :, 1:], res[1][:, :, 1:] * (FPEI_SOLVER2 / FPEI_CORE)) < 1e-9
     assert relcomp(e3[:, :, 1:], res[1][:, :, 1:]) < 1e-7
+    # FP32 convolution of the same rho (space_charge_3d at the cathode)
+    c32 = osc.Context(0, precision=osc.OSC_PREC_FP32)
+    m.rho = rho.copy(order="F")
+    osc.space_charge_3d(m, at_cathode=True, calc_bfield=True, ctx=c32)
+    c32.close()
+    err32 = relcomp(m.efield, e3)
+    print(f"  cathode FP32 vs FP64 {err32:.3e}", flush=True)
+    assert err32 < 1e-5 and relcomp(m.bfield, b3, 2) < 1e-5
 
 
 def test_config_pipe_walls_fullsize(ctx):
