@@ -204,11 +204,21 @@ __global__ void k_convert(const cpx<double>* __restrict__ in, cpx<float>* __rest
 // Axes along which the solve-path primitive of an E component is mirrored
 // (k_prim_offset): mev, where xlafun2 is even (its first argument), and mod,
 // where it is odd (its third argument); -1 where the axis has an offset or odd
-// length, and for phi or a Green function returned to the caller.
-static void prim_mirrors(const GreenSpec& s, int& mev, int& mod) {
+// length, for phi or a Green function returned to the caller, and unless
+// `fast` (the FP32 convolution).  The mirrored coordinate (i + 1/2 - L/2) d
+// is rounded once where upstream rounds (i d) and then its sum with
+// (1/2 - L/2) d; xlafun2's terms log(r + y) and log((r - z)/(r + z)) cancel
+// catastrophically where |y| or |z| dominates r, so that an ulp of coordinate
+// moves the primitive far more than an ulp of its value.  Measured on an
+// 8x8x256 mesh (512-point z lines): the mirrored build puts E 3e-11 of
+// max|E| from a build in upstream's arithmetic, which a restatement with
+// upstream's coordinates matches to 4e-14.  The FP64 path therefore
+// evaluates every point with upstream's arithmetic and mirrors nothing; the
+// FP32 convolution (rounding near 1e-7) keeps the quarter-grid evaluation.
+static void prim_mirrors(const GreenSpec& s, bool fast, int& mev, int& mod) {
   auto mirrorable = [&](int ax) { return s.offset[ax] == 0.0 && s.L[ax] % 2 == 0; };
   mev = mod = -1;
-  if (s.zero_raw && s.icomp >= 1) {
+  if (fast && s.zero_raw && s.icomp >= 1) {
     const int ea = s.icomp - 1, oa = (s.icomp + 1) % 3;
     if (mirrorable(ea)) mev = ea;
     if (mirrorable(oa)) mod = oa;
@@ -216,7 +226,7 @@ static void prim_mirrors(const GreenSpec& s, int& mev, int& mod) {
 }
 
 // primitive F of an offset-kind Green function (open_spacecharge_mod.f90:662-691)
-static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F) {
+static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F, bool fast) {
   const double dx = s.delta[0], dy = s.delta[1], dz = s.delta[2] * s.gamma;
   const double factor =
       (s.icomp == 1 || s.icomp == 2) ? s.gamma / (dx * dy * dz) : 1.0 / (dx * dy * dz);
@@ -227,12 +237,12 @@ static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F) {
   vmin = vmin + s.offset[1];
   wmin = wmin + s.offset[2] * s.gamma;
   const dim3 g3(unsigned((s.L[0] + 63) / 64), unsigned(s.L[1]), unsigned(s.L[2]));
-  // the solve path evaluates E components on a quarter of the grid (even
+  // the FP32 solve path evaluates E components on a quarter of the grid (even
   // mirror along their own axis, odd mirror along the axis of xlafun2's third
-  // argument, see k_prim_offset); the returned-to-caller Green function
-  // (zero_raw == 0) keeps upstream's coordinate arithmetic
+  // argument, see k_prim_offset); every other build keeps upstream's
+  // coordinate arithmetic on the whole grid
   int mev, mod;
-  prim_mirrors(s, mev, mod);
+  prim_mirrors(s, fast, mev, mod);
   dim3 gp = g3;
   if (mev == 0 || mod == 0) gp.x = unsigned((s.L[0] / 2 + 63) / 64);
   if (mev == 1 || mod == 1) gp.y = unsigned(s.L[1] / 2);
@@ -242,13 +252,13 @@ static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F) {
   OSC_HIP(hipGetLastError());
 }
 
-void build_green_real(osc_ctx* c, const GreenSpec& s, double* g) {
+void build_green_real(osc_ctx* c, const GreenSpec& s, double* g, bool fast) {
   const int64_t N = int64_t(s.L[0]) * s.L[1] * s.L[2];
   ProfScope ps(c, ST_GREEN_EVAL);
   if (s.kind == GK_OFFSET) {
     // osc_get_cgrn_freespace (open_spacecharge_mod.f90:633-707)
     DevBuf F(c->pool, sizeof(double) * N);
-    build_offset_primitive(c, s, F.as<double>());
+    build_offset_primitive(c, s, F.as<double>(), fast);
     const dim3 g3(unsigned((s.L[0] + 63) / 64), unsigned(s.L[1]), unsigned(s.L[2]));
     hipLaunchKernelGGL(k_diff_offset, g3, dim3(64), 0, c->stream, F.as<const double>(), g, s.L[0],
                        s.L[1], s.L[2], s.out_shift_z, s.zero_raw);
@@ -368,24 +378,25 @@ GreenRef green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) {
   DevBuf sp(c->pool, sizeof(cpx<T>) * d.size());
   GreenRef ref{};
   GreenRef* zr = fft_has_plan(s.L[2]) ? &ref : nullptr;
+  const bool fast = sizeof(T) == 4;  // mirrored quarter-grid primitive (prim_mirrors)
   if (s.kind == GK_OFFSET && fft_has_plan(s.L[0])) {
     // the x pass takes the 8-corner difference of the primitive as it loads
     // its rows, so the difference array is never stored
     {
       ProfScope ps(c, ST_GREEN_EVAL);
-      build_offset_primitive(c, s, g.as<double>());
+      build_offset_primitive(c, s, g.as<double>(), fast);
     }
     // the corner difference of a primitive that is even along an axis is odd
     // about L/2 - 1 there, and even where the primitive is odd
     int mev, mod;
-    prim_mirrors(s, mev, mod);
+    prim_mirrors(s, fast, mev, mod);
     int sym[3] = {0, 0, 0};
     if (mev >= 0) sym[mev] = -1;
     if (mod >= 0) sym[mod] = 1;
     const CornerDiff cd{s.L[1], s.L[2], s.out_shift_z, s.zero_raw};
     spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d, &cd, sym, zr);
   } else {
-    build_green_real(c, s, g.as<double>());
+    build_green_real(c, s, g.as<double>(), fast);
     spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d, nullptr, nullptr, zr);
   }
   g.reset();

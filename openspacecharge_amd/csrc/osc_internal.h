@@ -303,7 +303,7 @@ class HostRing {
 bool is_pageable(const void* p);
 
 // ---- green.hip --------------------------------------------------------------
-void build_green_real(osc_ctx* c, const GreenSpec& s, double* g);  // (L0,L1,L2) real
+void build_green_real(osc_ctx* c, const GreenSpec& s, double* g, bool fast = false);  // (L0,L1,L2) real
 // spectrum of a Green function in precision T (z-real when the z length has a
 // compile-time plan); cached by spec
 template <typename T>
