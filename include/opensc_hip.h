@@ -221,6 +221,21 @@ int osc_interpolate_field(osc_ctx* ctx, const osc_mesh* mesh, const double* efie
                           const double* bfield, int64_t n, const double* x, const double* y,
                           const double* z, double* E, double* B, int8_t* clamped,
                           int64_t* n_clamped);
+/* No upstream counterpart: osc_interpolate_field for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results bit-identical;
+ * every array must be device memory, and the mesh may have at most
+ * OSC_NOALLOC_MAX_NODES nodes (else OSC_ERR_ARG).  Its scratch (the packed
+ * node records, 64 MiB) is a fixed device array per GPU, held until the
+ * process ends and shared by every call on that GPU: calls that overlap on
+ * one GPU race and must be serialised by the caller.  Points are gathered in
+ * their own order (no cell sort), so very large batches are slower than
+ * osc_interpolate_field.  n_clamped, when given, waits for the result. */
+#define OSC_NOALLOC_MAX_NODES 1048576
+int osc_interpolate_field_noalloc(osc_ctx* ctx, const osc_mesh* mesh, const double* efield,
+                                  const double* bfield, int64_t n, const double* x,
+                                  const double* y, const double* z, double* E, double* B,
+                                  int8_t* clamped, int64_t* n_clamped);
 
 /* ---- request loop (no upstream counterpart) -----------------------------------
  * A stream of independent solves -- free space, cathode images (the

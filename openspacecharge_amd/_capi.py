@@ -27,6 +27,8 @@ OSC_ERR_NO_DEVICE = 10
 
 OSC_PREC_FP64 = 0
 OSC_PREC_FP32 = 1
+# most mesh nodes osc_interpolate_field_noalloc's fixed scratch holds
+OSC_NOALLOC_MAX_NODES = 1048576
 
 
 class LibraryNotBuilt(RuntimeError):
@@ -138,6 +140,8 @@ SIGNATURES = {
                                             C.c_int, _vp, _vp, _vp, _vp]),
     "osc_interpolate_field": (C.c_int, [_ctxp, _meshp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp,
                                         _vp, C.POINTER(_i64)]),
+    "osc_interpolate_field_noalloc": (C.c_int, [_ctxp, _meshp, _vp, _vp, _i64, _vp, _vp, _vp, _vp,
+                                                _vp, _vp, C.POINTER(_i64)]),
     "osc_server_create": (C.c_int, [C.c_int, C.c_int, _i64, C.c_int, C.POINTER(_vp)]),
     "osc_server_submit": (C.c_int, [_vp, C.c_void_p, C.POINTER(_i64)]),
     "osc_server_wait": (C.c_int, [_vp, _i64, _dp, C.POINTER(_i64)]),

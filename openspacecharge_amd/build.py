@@ -31,7 +31,8 @@ FLANG = os.path.join(ROCM, "bin", "amdflang")
 ARCH = os.environ.get("OSC_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["context.hip", "fft_f64.hip", "fft_f32.hip", "green.hip", "solve.hip",
-               "solvers.hip", "deposit.hip", "gather.hip", "capi.hip", "server.hip"]
+               "solvers.hip", "deposit.hip", "gather.hip", "capi.hip", "server.hip",
+               "noalloc.hip"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -95,6 +95,12 @@ OSC_API int osc_ctx_create(int device, osc_ctx** out) {
   c->hash_host = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(c->mesh_host) +
                                                        kHashHost);
   c->stream = c->own;
+  // the device address of the no-allocation gather's scratch on this device
+  const int rc = noalloc_setup(device, &c->last_error);
+  if (rc != OSC_OK) {
+    osc_ctx_destroy(c);
+    return rc;
+  }
   *out = c;
   return OSC_OK;
 }

@@ -498,4 +498,32 @@ int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const dou
   return 0;
 }
 
+void interpolate_packed(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
+                        int64_t n, const double* x, const double* y, const double* z, double* E,
+                        double* B, int8_t* clamped, double* rec, uint32_t* cnt) {
+  IP p{};
+  for (int d = 0; d < 3; ++d) {
+    p.mn[d] = a.mn[d];
+    p.delta[d] = a.delta[d];
+    p.hinv[d] = 1.0 / a.delta[d];  // hxi = 1.d0/delta (:319)
+    p.nlo[d] = a.nlo[d];
+    p.nhi[d] = a.nhi[d];
+    p.nn[d] = a.nn[d];
+    OSC_REQUIRE(a.nn[d] >= 2, OSC_ERR_ARG, "interpolation needs at least 2 nodes per axis");
+  }
+  fill_u32(c, cnt, 1, 0u);
+  if (n <= 0) return;
+  // no profiling scopes here: their events would be created on demand
+  const int64_t nt = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
+  const int64_t nbp = std::min<int64_t>((nt + kThreads - 1) / kThreads, 65536 * 8);
+  hipLaunchKernelGGL(k_pack_nodes, dim3(unsigned(nbp)), dim3(kThreads), 0, c->stream, ef, bf, nt,
+                     rec);
+  OSC_HIP(hipGetLastError());
+  const int64_t nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
+  hipLaunchKernelGGL(k_interp_rec<false>, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream, rec,
+                     int(E && ef), int(B && bf), n, x, y, z, static_cast<const double4*>(nullptr),
+                     static_cast<const uint32_t*>(nullptr), p, E, B, clamped, cnt);
+  OSC_HIP(hipGetLastError());
+}
+
 }  // namespace osc

@@ -137,9 +137,12 @@ struct MeshDev {
 };
 constexpr int kMeshDev = 256, kCounterBytes = 512;
 static_assert(kMeshDev + sizeof(MeshDev) <= kCounterBytes, "counter area too small");
-// the context's pinned host block: the mesh copy at 0, the two hashes at kHashHost
-constexpr int kHashHost = 128, kPinnedBytes = 256;
+// the context's pinned host block: the mesh copy at 0, the two hashes at
+// kHashHost, the clamped count of the no-allocation gather at kNoallocHost
+constexpr int kHashHost = 128, kNoallocHost = 192, kPinnedBytes = 256;
 static_assert(sizeof(MeshDev) <= kHashHost, "pinned block too small");
+static_assert(kHashHost + 16 <= kNoallocHost && kNoallocHost + 4 <= kPinnedBytes,
+              "pinned block layout");
 
 }  // namespace osc
 
@@ -386,6 +389,16 @@ struct InterpArgs {
 int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
                     int64_t n, const double* x, const double* y, const double* z, double* E,
                     double* B, int8_t* clamped);
+// the same gather with caller-provided device buffers and nothing allocated:
+// the node records of (ef, bf) are packed into rec (64 bytes per node), the
+// points are gathered in their own order, and the clamped count is left in
+// cnt (cleared first by a fill kernel)
+void interpolate_packed(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
+                        int64_t n, const double* x, const double* y, const double* z, double* E,
+                        double* B, int8_t* clamped, double* rec, uint32_t* cnt);
+// per-device address of the no-allocation gather's scratch array, looked up
+// once per device when a context is created on it (noalloc.hip)
+int noalloc_setup(int device, std::string* err);
 // same, for the particles of the preceding deposit in their cell-sorted order;
 // packed: the node records of (ef, bf) already queued by pack_nodes, or null
 void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,

@@ -84,6 +84,8 @@ def test_constants_match_header(lib):
     assert lib.osc_abi_version() == ver
     nst = int(re.search(r"#define OSC_NSTAGE (\d+)", txt).group(1))
     assert _capi.Context.NSTAGE == nst
+    mx = int(re.search(r"#define OSC_NOALLOC_MAX_NODES (\d+)", txt).group(1))
+    assert _capi.OSC_NOALLOC_MAX_NODES == mx
 
 
 def test_mesh_struct_layout(tmp_path):
