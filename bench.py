@@ -63,6 +63,8 @@ def parse():
                    help="bunches of the batched mixed-grid leg (0 skips it)")
     p.add_argument("--served-requests", type=int, default=64,
                    help="requests of the served mixed stream (0 skips it)")
+    p.add_argument("--poisson-requests", type=int, default=1000,
+                   help="requests of each Poisson-arrival run of the served mix (three loads)")
     return p.parse_args()
 
 
@@ -292,20 +294,78 @@ def make_stream(kind, count, seed):
 def run_stream(reqs, parts, nworkers, precision, green_cache, sequential, rate=None, seed=1,
                warm=None):
     """Runs the requests through a Server; returns (wall s, per-request
-    records, outputs).  sequential: each request waits for the previous one
-    (plain per-request solving).  rate: Poisson arrivals at this many
-    requests/s (otherwise all at once).  warm: (requests, bunches) of another
-    seed run first on the same server, untimed, so that the timed stream finds
-    the device buffer pools of a running server (its meshes differ, so it
-    leaves no Green spectra the timed stream could reuse)."""
-    import torch
+    records, outputs, server counters of the timed stream).  sequential: each
+    request waits for the previous one (plain per-request solving).  rate:
+    Poisson arrivals at this many requests/s (otherwise all at once).  warm:
+    (requests, bunches) of another seed run first on the same server, untimed,
+    so that the timed stream finds the device buffer pools of a running server
+    (its meshes differ, so it leaves no Green spectra the timed stream could
+    reuse)."""
     import openspacecharge_amd as osc
     srv = osc.Server(nworkers=nworkers, green_cache=green_cache)
     if warm is not None:
         _drive(srv, warm[0], warm[1], precision, sequential)
+    st0 = srv.stats()
     wall, recs, outs = _drive(srv, reqs, parts, precision, sequential, rate, seed)
+    st1 = srv.stats()
     srv.close()
-    return wall, recs, outs
+    st = {k: st1[k] - st0[k] for k in ("done", "affine_starts", "bypasses", "green_hits",
+                                       "green_misses")}
+    st["max_running"] = st1["max_running"]
+    return wall, recs, outs, st
+
+
+def long_stream(reqs, parts, nreq, rate, precision, nworkers=4, seed=5):
+    """nreq requests with Poisson arrivals at `rate`/s: request k is request
+    k % len(reqs) with its bunch stretched by its own factors in [0.9, 1.1]
+    per axis (z only, and the fixed mesh with it, for the pipe), so that
+    every mesh differs and no Green spectrum is reused.  Requests are waited
+    for in submission order by a second thread, which drops them once done.
+    Returns (wall s, latencies ms, server counters)."""
+    import threading
+    import queue as qmod
+    import torch
+    import openspacecharge_amd as osc
+    srv = osc.Server(nworkers=nworkers, green_cache=True)
+    rng = np.random.Generator(np.random.Philox(seed))
+    pend = qmod.Queue()
+    lat = []
+
+    def reaper():
+        while True:
+            item = pend.get()
+            if item is None:
+                return
+            lat.append(srv.wait(item[0])["total_ms"])
+
+    th = threading.Thread(target=reaper)
+    th.start()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t_next = t0
+    for k in range(nreq):
+        r = reqs[k % len(reqs)]
+        x, y, z = parts[k % len(reqs)]
+        f = rng.uniform(0.9, 1.1, 3)
+        kw = dict(r["kw"])
+        if kw["solver"] == "pipe":
+            f[0] = f[1] = 1.0
+            kw["mesh_min"] = tuple(kw["mesh_min"][:2]) + (kw["mesh_min"][2] * f[2],)
+            kw["mesh_max"] = tuple(kw["mesh_max"][:2]) + (kw["mesh_max"][2] * f[2],)
+        xs, ys, zs = x * float(f[0]), y * float(f[1]), z * float(f[2])
+        E = torch.empty((x.shape[0], 3), dtype=torch.float64, device="cuda")
+        torch.cuda.current_stream().synchronize()  # the stretched copies exist before submit
+        t_next += rng.exponential(1.0 / rate)
+        while time.perf_counter() < t_next:
+            time.sleep(max(0.0, min(1e-3, t_next - time.perf_counter())))
+        tk = srv.submit(xs, ys, zs, E, precision=precision, **kw)
+        pend.put((tk, xs, ys, zs, E))
+    pend.put(None)
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return wall, lat, st
 
 
 def _drive(srv, reqs, parts, precision, sequential, rate=None, seed=1):
@@ -348,10 +408,11 @@ def stream_legs(args):
         warm = (wreqs, wparts)
         torch.cuda.synchronize()
         # plain FP64 per request: one at a time, the Green function rebuilt for every request
-        w64, _, o64 = run_stream(reqs, parts, 1, osc.OSC_PREC_FP64, False, True, warm=warm)
+        w64, _, o64, _ = run_stream(reqs, parts, 1, osc.OSC_PREC_FP64, False, True, warm=warm)
         # served: concurrent workers, Green spectra kept resident
-        ws64, _, _ = run_stream(reqs, parts, 4, osc.OSC_PREC_FP64, True, False, warm=warm)
-        ws32, recs32, o32 = run_stream(reqs, parts, 4, osc.OSC_PREC_FP32, True, False, warm=warm)
+        ws64, _, _, st64 = run_stream(reqs, parts, 4, osc.OSC_PREC_FP64, True, False, warm=warm)
+        ws32, recs32, o32, st32 = run_stream(reqs, parts, 4, osc.OSC_PREC_FP32, True, False,
+                                             warm=warm)
         err = 0.0
         for a, b in zip(o32, o64):
             a, b = a.cpu().numpy(), b.cpu().numpy()
@@ -362,17 +423,25 @@ def stream_legs(args):
                "fp64_served_solves_per_s": round(count / ws64, 2),
                "fp32_served_solves_per_s": round(count / ws32, 2),
                "fp32_served_vs_fp64_per_request": round(w64 / ws32, 3),
-               "max_rel_err_fp32_vs_fp64": err}
+               "target_3x": "pass" if w64 / ws32 >= 3.0 else "fail",
+               "max_rel_err_fp32_vs_fp64": err,
+               "server_fp64": st64, "server_fp32": st32}
         if kind == "served":
             out["mix"] = "70% free / 20% cathode / 10% pipe, grid edge 32..256 (2^a 3^b), 1e5-2e7 particles"
-            rate = 0.7 * count / ws32
-            wl, recl, _ = run_stream(reqs, parts, 4, osc.OSC_PREC_FP32, True, False, rate=rate,
-                                     warm=warm)
-            lat = [r["total_ms"] for r in recl]
-            out.update({"poisson_rate_per_s": round(rate, 2),
-                        "fp32_p50_ms": round(float(np.percentile(lat, 50)), 3),
-                        "fp32_p99_ms": round(float(np.percentile(lat, 99)), 3),
-                        "fp32_rate_solves_per_s": round(count / wl, 2)})
+            cap = count / ws32
+            poisson = []
+            for load in (0.5, 0.7, 0.9):
+                wl, lat, stl = long_stream(reqs, parts, args.poisson_requests, load * cap,
+                                           osc.OSC_PREC_FP32)
+                poisson.append({"load": load, "rate_per_s": round(load * cap, 2),
+                                "requests": len(lat), "wall_s": round(wl, 2),
+                                "solves_per_s": round(len(lat) / wl, 2),
+                                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                                "max_running": stl["max_running"], "bypasses": stl["bypasses"],
+                                "green_hits": stl["green_hits"]})
+                log(f"poisson {load}: {poisson[-1]}")
+            out["fp32_poisson"] = poisson
         else:
             out["mix"] = "free space, grids {32^3,48^3,64^3,96x96x128,128^3}, 1e5-4e6 particles"
         res[kind] = out

@@ -243,12 +243,16 @@ int osc_interpolate_field_noalloc(osc_ctx* ctx, const osc_mesh* mesh, const doub
  * deposit -> solve -> gather of E (and B) at its particles, as a tracking code
  * would call deposit_particles, the solver and interpolate_field.  nworkers
  * host threads each own a context and HIP stream, so requests overlap on the
- * GPU; requests with the same solver, mesh shape, fixed mesh geometry, gamma
- * and pipe size go to the same worker and reuse its cached Green spectra
- * (green_cache != 0).  A request starts when its estimated device bytes fit
- * the budget next to the running ones (max_bytes, or 7/8 of the free memory
- * when max_bytes <= 0).  Particle and output arrays are device memory and
- * must stay valid until osc_server_wait returns for the request. */
+ * GPU.  Requests wait in one queue; an idle worker starts the oldest one that
+ * fits the memory budget, preferring one whose geometry (solver, mesh shape,
+ * fixed mesh, gamma, pipe size) it served recently, so that its cached Green
+ * spectra are reused (green_cache != 0).  The budget (max_bytes, or 7/8 of the
+ * free memory when max_bytes <= 0) covers the estimated bytes of the running
+ * requests and what the workers' contexts hold between requests (Green
+ * caches, capped at budget/(8 nworkers) each, pipe Green functions, pooled
+ * buffers); a request larger than the budget runs alone.  Particle and output
+ * arrays are device memory and must stay valid until osc_server_wait returns
+ * for the request. */
 enum osc_solver { OSC_SOLVE_FREE = 0, OSC_SOLVE_CATHODE = 1, OSC_SOLVE_PIPE = 2 };
 typedef struct osc_request {
   int32_t solver;          /* osc_solver                                             */
@@ -272,6 +276,13 @@ int osc_server_submit(osc_server* server, const osc_request* req, int64_t* ticke
  * deposit, points clamped by the gather.  Returns the request's status. */
 int osc_server_wait(osc_server* server, int64_t ticket, double* times_ms, int64_t* counts);
 int osc_server_drain(osc_server* server);
+/* counters since creation, the first n of: requests done, most requests
+ * running at once, requests started on a worker that served the same geometry
+ * before, requests started ahead of an older one that did not fit, Green
+ * spectrum cache hits, misses (over all workers), bytes the idle workers'
+ * contexts hold, the byte budget */
+#define OSC_SERVER_NSTATS 8
+int osc_server_stats(osc_server* server, int64_t* out, int n);
 const char* osc_server_last_error(const osc_server* server);
 int osc_server_destroy(osc_server* server);
 

@@ -29,6 +29,7 @@ OSC_PREC_FP64 = 0
 OSC_PREC_FP32 = 1
 # most mesh nodes osc_interpolate_field_noalloc's fixed scratch holds
 OSC_NOALLOC_MAX_NODES = 1048576
+OSC_SERVER_NSTATS = 8
 
 
 class LibraryNotBuilt(RuntimeError):
@@ -146,6 +147,7 @@ SIGNATURES = {
     "osc_server_submit": (C.c_int, [_vp, C.c_void_p, C.POINTER(_i64)]),
     "osc_server_wait": (C.c_int, [_vp, _i64, _dp, C.POINTER(_i64)]),
     "osc_server_drain": (C.c_int, [_vp]),
+    "osc_server_stats": (C.c_int, [_vp, C.POINTER(_i64), C.c_int]),
     "osc_server_last_error": (C.c_char_p, [_vp]),
     "osc_server_destroy": (C.c_int, [_vp]),
 }

@@ -524,9 +524,9 @@ _SOLVERS = {"free": K.OSC_SOLVE_FREE, "cathode": K.OSC_SOLVE_CATHODE, "pipe": K.
 class Server:
     """A stream of solves (free space, cathode images, rectangular pipe), each
     deposit -> solve -> gather at its particles.  ``nworkers`` contexts on
-    their own HIP streams run requests concurrently; requests with the same
-    geometry share a worker and its cached Green spectra; a request starts
-    when its estimated device bytes fit the budget next to the running ones.
+    their own HIP streams run requests concurrently; an idle worker starts the
+    oldest queued request that fits the memory budget, preferring one whose
+    geometry it served recently (its cached Green spectra).
     Particle and output arrays are torch tensors on the GPU and are kept
     alive by the server until ``wait`` returns for the request."""
 
@@ -587,6 +587,17 @@ class Server:
 
     def drain(self) -> None:
         self.lib.osc_server_drain(self.h)
+
+    STATS = ("done", "max_running", "affine_starts", "bypasses", "green_hits", "green_misses",
+             "held_bytes", "budget_bytes")
+
+    def stats(self) -> dict:
+        """Counters since creation (osc_server_stats)."""
+        v = (C.c_int64 * K.OSC_SERVER_NSTATS)()
+        rc = self.lib.osc_server_stats(self.h, v, K.OSC_SERVER_NSTATS)
+        if rc != K.OSC_OK:
+            raise K.OSCError(rc, "osc_server_stats failed")
+        return dict(zip(self.STATS, list(v)))
 
     def close(self) -> None:
         if getattr(self, "h", None):

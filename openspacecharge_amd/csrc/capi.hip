@@ -313,7 +313,10 @@ OSC_API int osc_deposit_particles(osc_ctx* c, int64_t n, const double* xa, const
     double* r = (qa || total_charge) ? st.out(rho, size_t(prod3(nn))) : nullptr;
     // a large deposit keeps its cell order and the content hash of the points
     // for an interpolate_field at the same points (the usual next call but one)
+    // the previous kept order goes back to the pool first, so that a new
+    // large deposit never holds two copies
     c->kept_ok = false;
+    c->kept = SortedParticles{};
     const bool keep = r && n >= kGatherSortMin;
     unsigned long long* hash = reinterpret_cast<unsigned long long*>(
         static_cast<char*>(c->counters) + kHashDeposit);

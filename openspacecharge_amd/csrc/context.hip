@@ -31,6 +31,7 @@ void* DevPool::get(size_t bytes) {
     free_.erase(it);
     cached_ -= have;
     live_[p] = have;
+    live_bytes_ += have;
     return p;
   }
   void* p = nullptr;
@@ -46,6 +47,7 @@ void* DevPool::get(size_t bytes) {
     }
   }
   live_[p] = bb;
+  live_bytes_ += bb;
   return p;
 }
 
@@ -54,6 +56,7 @@ void DevPool::put(void* p) {
   if (it == live_.end()) return;
   size_t bb = it->second;
   live_.erase(it);
+  live_bytes_ -= bb;
   free_.emplace(bb, p);
   cached_ += bb;
 }
@@ -68,6 +71,7 @@ void DevPool::release_all() {
   release_free();
   for (auto& kv : live_) (void)hipFree(kv.first);
   live_.clear();
+  live_bytes_ = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -390,10 +394,13 @@ GreenSpec make_green_spec() {
 }
 
 void GreenCache::evict_to(size_t budget) {
-  while (bytes > budget && !map.empty()) {
-    auto victim = map.begin();
+  while (bytes > budget) {
+    auto victim = map.end();
     for (auto it = map.begin(); it != map.end(); ++it)
-      if (it->second->last < victim->second->last) victim = it;
+      if (it->second->last < pin_from &&
+          (victim == map.end() || it->second->last < victim->second->last))
+        victim = it;
+    if (victim == map.end()) return;  // everything left is in use
     bytes -= victim->second->bytes;
     map.erase(victim);
   }

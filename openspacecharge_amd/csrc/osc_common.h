@@ -87,12 +87,13 @@ class DevPool {
   void release_free();
   void release_all();
   size_t bytes_cached() const { return cached_; }
+  size_t bytes_live() const { return live_bytes_; }
   ~DevPool() { release_all(); }
 
  private:
   std::multimap<size_t, void*> free_;  // by size: best fit
   std::unordered_map<void*, size_t> live_;
-  size_t cached_ = 0;
+  size_t cached_ = 0, live_bytes_ = 0;
 };
 
 // RAII handle on a pooled device buffer

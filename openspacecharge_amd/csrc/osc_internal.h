@@ -67,6 +67,18 @@ class GreenCache {
   uint64_t tick = 0;
   std::unordered_map<std::string, std::unique_ptr<Entry>> map;
   uint64_t hits = 0, misses = 0;
+  // entries used since pin() (last >= pin_from) are not evicted until the
+  // matching unpin(): a solve holds references to all its spectra until its
+  // kernels are queued, and an evicted buffer could be reused for the next
+  // spectrum before that
+  uint64_t pin_from = ~uint64_t(0);
+  int pin_depth = 0;
+  void pin() {
+    if (pin_depth++ == 0) pin_from = tick + 1;
+  }
+  void unpin() {
+    if (--pin_depth == 0) pin_from = ~uint64_t(0);
+  }
   void clear() {
     map.clear();
     bytes = 0;

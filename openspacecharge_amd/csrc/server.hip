@@ -4,12 +4,19 @@
 // Workers: each owns an osc_ctx with its own HIP stream, buffer pool and
 // Green-spectrum cache, and a host thread that runs one request at a time
 // through the C-ABI (deposit -> solve -> gather), so requests on different
-// workers overlap on the device.  Requests are routed by a key of (solver,
-// mesh shape, mesh geometry when fixed, gamma, pipe size) so that repeated
-// geometries meet their cached Green spectra.  Admission: a request starts
-// only when the estimated device bytes of the requests running plus its own
-// fit the byte budget (free HBM at creation less a margin, or the caller's
-// limit); a request that alone exceeds the budget runs when nothing else does.
+// workers overlap on the device.  Admission is work-conserving: requests wait
+// in one queue, oldest first, and an idle worker takes the oldest request
+// that fits the memory budget, preferring one (among the first 2 x nworkers)
+// whose geometry key it served recently, since its context holds that Green
+// spectrum.  A request that does not fit may be passed over by smaller ones
+// at most kMaxBypass times; after that nothing starts before it.  Memory
+// accounting: the budget (free HBM at creation less an eighth, or the
+// caller's limit) covers the estimated bytes of the running requests plus
+// what every worker's context holds between requests (Green caches, the pipe
+// Green functions, pooled buffers); each worker's Green cache is capped at
+// budget / (8 nworkers) and its pooled buffers are given back when it holds
+// more than budget / (2 nworkers) after a request.  A request larger than the
+// budget runs alone.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -30,8 +37,15 @@ struct Job {
   int64_t ticket;
   osc_request r;
   int64_t est;
+  uint64_t key;
+  int bypassed = 0;
   Clock::time_point t_submit;
 };
+
+// a request that does not fit may be passed over this many times
+constexpr int kMaxBypass = 8;
+// route keys a worker remembers as served (its likely cached Green spectra)
+constexpr size_t kAffinityKeys = 16;
 
 struct Done {
   int status = OSC_OK;
@@ -49,7 +63,7 @@ uint64_t dbits(double d) {
   return u;
 }
 
-// routing key: requests that can share Green spectra share a worker
+// geometry key: requests that can share Green spectra
 uint64_t route_key(const osc_request& r) {
   uint64_t h = mix(0, uint64_t(r.solver));
   for (int a = 0; a < 3; ++a) h = mix(h, uint64_t(r.nhi[a]));
@@ -75,9 +89,14 @@ struct osc_server {
   int device = 0;
   int green_cache = 1;
   int64_t budget = 0, reserved = 0;
+  int running = 0;
   std::vector<osc_ctx*> ctxs;
   std::vector<std::thread> threads;
-  std::vector<std::deque<Job>> queues;
+  std::deque<Job> queue;                    // admission order, oldest first
+  std::vector<std::deque<uint64_t>> affinity;  // per worker: keys served recently
+  std::vector<int64_t> held;                // per worker: bytes its context holds when idle
+  int64_t held_total = 0;
+  int64_t n_done = 0, max_running = 0, n_affine = 0, n_bypass = 0;
   std::unordered_map<int64_t, Done> done;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -88,7 +107,38 @@ struct osc_server {
 
   void run_worker(int w);
   int run_one(osc_ctx* c, const osc_request& r, Done& d);
+  // index in `queue` of the request worker w should start now, or -1
+  int pick(int w);
+  void set_held(int w, int64_t bytes);
 };
+
+int osc_server::pick(int w) {
+  if (queue.empty()) return -1;
+  auto fits = [&](const Job& j) {
+    return running == 0 || held_total + reserved + j.est <= budget;
+  };
+  const auto& aff = affinity[w];
+  auto mine = [&](uint64_t k) {
+    for (uint64_t a : aff)
+      if (a == k) return true;
+    return false;
+  };
+  const size_t look = std::min(queue.size(), 2 * ctxs.size());
+  for (size_t i = 0; i < look; ++i)
+    if (mine(queue[i].key) && fits(queue[i]) && (i == 0 || fits(queue[0]) ||
+                                                  queue[0].bypassed < kMaxBypass))
+      return int(i);
+  if (fits(queue[0])) return 0;
+  if (queue[0].bypassed >= kMaxBypass) return -1;  // the head waits for memory
+  for (size_t i = 1; i < queue.size(); ++i)
+    if (fits(queue[i])) return int(i);
+  return -1;
+}
+
+void osc_server::set_held(int w, int64_t bytes) {
+  held_total += bytes - held[w];
+  held[w] = bytes;
+}
 
 int osc_server::run_one(osc_ctx* c, const osc_request& r, Done& d) {
   using namespace osc;
@@ -140,6 +190,9 @@ int osc_server::run_one(osc_ctx* c, const osc_request& r, Done& d) {
     if (rc) return rc;
     rc = osc_interpolate_field(c, &m, ef.as<double>(), want_b ? bf.as<double>() : nullptr, r.n,
                                r.x, r.y, r.z, r.E, r.B, nullptr, &d.n_clamped);
+    // the deposit's kept cell order has served its gather: back to the pool
+    c->kept_ok = false;
+    c->kept = SortedParticles{};
     if (rc) return rc;
   } else {
     c->last_error = "unknown solver";
@@ -150,20 +203,31 @@ int osc_server::run_one(osc_ctx* c, const osc_request& r, Done& d) {
 
 void osc_server::run_worker(int w) {
   osc_ctx* c = ctxs[w];
+  // freeing pooled buffers (hipFree) stalls the device and the next request
+  // then allocates again: only a context holding more than this gives back
+  const int64_t share = budget / int64_t(2 * ctxs.size());
   for (;;) {
     Job job;
     {
       std::unique_lock<std::mutex> lk(mu);
+      int at = -1;
+      // after stop, the queued requests are still served
       cv_work.wait(lk, [&] {
-        if (stop) return true;
-        if (queues[w].empty()) return false;
-        const int64_t est = queues[w].front().est;
-        return reserved == 0 || reserved + est <= budget;
+        if (stop && queue.empty()) return true;
+        at = pick(w);
+        return at >= 0;
       });
-      if (stop && queues[w].empty()) return;
-      job = queues[w].front();
-      queues[w].pop_front();
+      if (at < 0) return;
+      job = queue[size_t(at)];
+      if (at > 0) {
+        ++queue[0].bypassed;
+        ++n_bypass;
+      }
+      for (uint64_t a : affinity[w]) n_affine += a == job.key ? 1 : 0;
+      queue.erase(queue.begin() + at);
       reserved += job.est;
+      ++running;
+      max_running = std::max<int64_t>(max_running, running);
     }
     const auto t_start = Clock::now();
     Done d;
@@ -172,13 +236,25 @@ void osc_server::run_worker(int w) {
     d.queued_ms = std::chrono::duration<double, std::milli>(t_start - job.t_submit).count();
     d.run_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
     d.total_ms = std::chrono::duration<double, std::milli>(t_end - job.t_submit).count();
-    // pooled buffers beyond a share of the budget go back to the device
-    if (int64_t(c->pool.bytes_cached()) > budget / int64_t(2 * ctxs.size())) c->pool.release_free();
+    // what the context keeps between requests: Green cache, pipe Green
+    // functions and pooled buffers; pooled buffers beyond the share go back
+    if (int64_t(c->pool.bytes_cached() + c->pool.bytes_live()) > share) c->pool.release_free();
+    const int64_t keep = int64_t(c->pool.bytes_cached() + c->pool.bytes_live());
     {
       std::lock_guard<std::mutex> lk(mu);
       if (d.status != OSC_OK) last_error = c->last_error;
       reserved -= job.est;
+      --running;
+      set_held(w, keep);
+      auto& aff = affinity[w];
+      bool have = false;
+      for (uint64_t a : aff) have = have || a == job.key;
+      if (!have) {
+        aff.push_back(job.key);
+        if (aff.size() > kAffinityKeys) aff.pop_front();
+      }
       --in_flight;
+      ++n_done;
       done[job.ticket] = d;
     }
     cv_work.notify_all();
@@ -201,7 +277,6 @@ OSC_API int osc_server_create(int device, int nworkers, int64_t max_bytes, int g
       delete s;
       return rc;
     }
-    osc_ctx_set_green_cache(c, green_cache, 0);
     s->ctxs.push_back(c);
   }
   size_t fr = 0, tot = 0;
@@ -214,7 +289,10 @@ OSC_API int osc_server_create(int device, int nworkers, int64_t max_bytes, int g
   }
   const int64_t avail = int64_t(fr) - int64_t(fr / 8);  // keep an eighth free
   s->budget = max_bytes > 0 ? std::min<int64_t>(max_bytes, avail) : avail;
-  s->queues.resize(nworkers);
+  for (auto* c : s->ctxs)
+    osc_ctx_set_green_cache(c, green_cache, std::max<int64_t>(1, s->budget / (8 * nworkers)));
+  s->affinity.resize(nworkers);
+  s->held.assign(nworkers, 0);
   for (int w = 0; w < nworkers; ++w) s->threads.emplace_back([s, w] { s->run_worker(w); });
   *out = s;
   return OSC_OK;
@@ -228,14 +306,14 @@ OSC_API int osc_server_submit(osc_server* s, const osc_request* r, int64_t* tick
   Job j;
   j.r = *r;
   j.est = estimate_bytes(*r);
+  j.key = route_key(*r);
   j.t_submit = Clock::now();
-  const size_t w = size_t(route_key(*r) % s->ctxs.size());
   {
     std::lock_guard<std::mutex> lk(s->mu);
     j.ticket = s->next_ticket++;
     *ticket = j.ticket;
     ++s->in_flight;
-    s->queues[w].push_back(j);
+    s->queue.push_back(j);
   }
   s->cv_work.notify_all();
   return OSC_OK;
@@ -264,6 +342,20 @@ OSC_API int osc_server_drain(osc_server* s) {
   if (!s) return OSC_ERR_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
   s->cv_done.wait(lk, [&] { return s->in_flight == 0; });
+  return OSC_OK;
+}
+
+OSC_API int osc_server_stats(osc_server* s, int64_t* out, int n) {
+  if (!s || !out || n < 1) return OSC_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  int64_t hits = 0, misses = 0;
+  for (auto* c : s->ctxs) {
+    hits += int64_t(c->gcache.hits);
+    misses += int64_t(c->gcache.misses);
+  }
+  const int64_t v[OSC_SERVER_NSTATS] = {s->n_done,     s->max_running, s->n_affine, s->n_bypass,
+                                        hits,          misses,         s->held_total, s->budget};
+  for (int i = 0; i < n && i < OSC_SERVER_NSTATS; ++i) out[i] = v[i];
   return OSC_OK;
 }
 

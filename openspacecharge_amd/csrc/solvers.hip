@@ -72,11 +72,23 @@ GreenSpec legacy_spec(const int nlo[3], const int nhi[3], const int npad[3], con
   return s;
 }
 
+// the Green spectra of one solve: cached ones stay pinned in the context's
+// cache until the solve's kernels are queued (the end of its scope)
 template <typename T>
 struct Specs {
   std::vector<DevBuf> keep;
+  osc_ctx* pinned = nullptr;
   explicit Specs(size_t n) : keep(n) {}
+  ~Specs() {
+    if (pinned) pinned->gcache.unpin();
+  }
+  Specs(const Specs&) = delete;
+  Specs& operator=(const Specs&) = delete;
   GreenRef get(osc_ctx* c, const GreenSpec& s, size_t slot) {
+    if (!pinned) {
+      pinned = c;
+      c->gcache.pin();
+    }
     return green_spectrum<T>(c, s, keep[slot]);
   }
 };

@@ -86,6 +86,8 @@ def test_constants_match_header(lib):
     assert _capi.Context.NSTAGE == nst
     mx = int(re.search(r"#define OSC_NOALLOC_MAX_NODES (\d+)", txt).group(1))
     assert _capi.OSC_NOALLOC_MAX_NODES == mx
+    ns = int(re.search(r"#define OSC_SERVER_NSTATS (\d+)", txt).group(1))
+    assert _capi.OSC_SERVER_NSTATS == ns
 
 
 def test_mesh_struct_layout(tmp_path):

@@ -1,8 +1,8 @@
 """The request loop (osc_server_*): results are the same alone and in a mix of
 concurrent requests (the kernels are deterministic, so bit for bit), equal
-to the single-call API, FP32 requests stay within 1e-5 of FP64, the memory
-budget admits requests one at a time when it must, and errors are reported
-per request."""
+to the single-call API, FP32 requests stay within 1e-5 of FP64, requests of
+one geometry spread over the workers, the memory budget admits requests one
+at a time when it must, and errors are reported per request."""
 import numpy as np
 import pytest
 
@@ -112,6 +112,29 @@ def test_server_tiny_budget_serialises():
     ref_srv.close()
     for (E, B), (Er, Br) in zip(res, ref):
         assert np.array_equal(E, Er)
+
+
+def test_server_is_work_conserving():
+    """Requests of one geometry are not tied to one worker: with 4 workers they
+    run concurrently, later ones on workers that served the geometry before
+    (their Green spectra cached), and the results stay bit-identical to one
+    worker running them in order."""
+    import torch
+    (t, kw) = _mix()[1]
+    kw = dict(kw, mesh_min=(-2e-3, -2e-3, -8e-3), mesh_max=(2e-3, 2e-3, 8e-3))
+    reqs = [(t, kw)] * 12
+    srv = osc.Server(nworkers=4)
+    res = _run(srv, reqs, osc.OSC_PREC_FP64, concurrent=True)
+    st = srv.stats()
+    srv.close()
+    assert st["done"] == 12 and st["max_running"] >= 2, st
+    # E x, y, z spectra: at most one build of each per worker
+    assert st["green_misses"] <= 3 * 4 and st["green_hits"] >= 3 * (12 - 4), st
+    ref_srv = osc.Server(nworkers=1)
+    ref = _run(ref_srv, reqs[:1], osc.OSC_PREC_FP64, concurrent=False)
+    ref_srv.close()
+    for E, B in res:
+        assert np.array_equal(E, ref[0][0]) and np.array_equal(B, ref[0][1])
 
 
 def test_server_reports_request_errors():
