@@ -78,6 +78,20 @@ int osc_ctx_set_async(osc_ctx* ctx, int async);
 int osc_ctx_set_precision(osc_ctx* ctx, int precision);
 /* Green-spectrum cache: enable (default 1) and byte budget (default 8 GiB) */
 int osc_ctx_set_green_cache(osc_ctx* ctx, int enable, int64_t max_bytes);
+/* Fused steps (osc_space_charge_step) with a resized (bunch-following) mesh
+ * of at most 128^3 nodes and no offset record their grid part -- Green
+ * build, spectral solve, B -- as a HIP graph after the first step of each
+ * (shape, precision, solver, B, gamma), and replay it for later steps of
+ * that shape whatever their mesh: the Green primitive reads the spacing from
+ * the device, so the solve is queued without waiting for the mesh.  Results
+ * are bit-identical to the eager launches.  Fixed meshes run eagerly and
+ * reuse cached Green spectra.  Default off (measured on MI355X: replay is
+ * 2-7 % slower than the eager launches from 32^3 to 128^3, whose host work
+ * already overlaps the device); off clears the graphs.  Profiling
+ * (osc_ctx_set_profiling) runs the eager launches. */
+int osc_ctx_set_graphs(osc_ctx* ctx, int enable);
+/* graphs held, graphs captured, replays since creation */
+int osc_ctx_graph_stats(osc_ctx* ctx, int64_t* out3);
 int osc_ctx_clear_caches(osc_ctx* ctx);
 int osc_ctx_synchronize(osc_ctx* ctx);
 const char* osc_ctx_last_error(const osc_ctx* ctx);

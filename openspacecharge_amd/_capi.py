@@ -102,6 +102,8 @@ SIGNATURES = {
     "osc_ctx_set_async": (C.c_int, [_ctxp, C.c_int]),
     "osc_ctx_set_precision": (C.c_int, [_ctxp, C.c_int]),
     "osc_ctx_set_green_cache": (C.c_int, [_ctxp, C.c_int, _i64]),
+    "osc_ctx_set_graphs": (C.c_int, [_ctxp, C.c_int]),
+    "osc_ctx_graph_stats": (C.c_int, [_ctxp, C.POINTER(_i64)]),
     "osc_ctx_clear_caches": (C.c_int, [_ctxp]),
     "osc_ctx_synchronize": (C.c_int, [_ctxp]),
     "osc_ctx_last_error": (C.c_char_p, [_ctxp]),
@@ -229,6 +231,15 @@ class Context:
 
     def clear_caches(self):
         self.check(self.lib.osc_ctx_clear_caches(self.h))
+
+    def set_graphs(self, enable: bool):
+        """Captured-solve replay of fused steps (osc_ctx_set_graphs)."""
+        self.check(self.lib.osc_ctx_set_graphs(self.h, int(bool(enable))))
+
+    def graph_stats(self) -> dict:
+        v = (C.c_int64 * 3)()
+        self.check(self.lib.osc_ctx_graph_stats(self.h, v))
+        return dict(held=v[0], captured=v[1], replays=v[2])
 
     def synchronize(self):
         self.check(self.lib.osc_ctx_synchronize(self.h))

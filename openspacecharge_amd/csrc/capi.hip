@@ -113,6 +113,7 @@ OSC_API int osc_ctx_destroy(osc_ctx* c) {
     if (c->stream != c->own) (void)hipStreamSynchronize(c->own);
     c->ring.reset();
     c->kept = SortedParticles{};
+    c->sgraphs.clear();
     c->gcache.clear();
     for (int t = 0; t < 4; ++t) {
       c->pipe.spec64[t].reset();
@@ -167,11 +168,29 @@ OSC_API int osc_ctx_set_green_cache(osc_ctx* c, int enable, int64_t max_bytes) {
   });
 }
 
+OSC_API int osc_ctx_set_graphs(osc_ctx* c, int enable) {
+  return guarded(c, [&] {
+    OSC_HIP(hipStreamSynchronize(c->stream));
+    c->graphs = enable != 0;
+    if (!c->graphs) c->sgraphs.clear();
+    return OSC_OK;
+  });
+}
+
+OSC_API int osc_ctx_graph_stats(osc_ctx* c, int64_t* out3) {
+  if (!c || !out3) return OSC_ERR_ARG;
+  out3[0] = int64_t(c->sgraphs.size());
+  out3[1] = c->graph_builds;
+  out3[2] = c->graph_replays;
+  return OSC_OK;
+}
+
 OSC_API int osc_ctx_clear_caches(osc_ctx* c) {
   return guarded(c, [&] {
     OSC_HIP(hipStreamSynchronize(c->stream));
     c->kept = SortedParticles{};
     c->kept_ok = false;
+    c->sgraphs.clear();
     c->gcache.clear();
     c->pool.release_free();
     return OSC_OK;
@@ -233,10 +252,109 @@ OSC_API int osc_ctx_profile_read(osc_ctx* c, double* ms, int64_t* launches) {
 // deposition
 // ---------------------------------------------------------------------------
 // deposit_particles body on staged (device) coordinates; rho is device memory
+// ---------------------------------------------------------------------------
+// captured solves of the fused step
+// ---------------------------------------------------------------------------
+namespace {
+MeshDev* mesh_dev(osc_ctx* c) {
+  return reinterpret_cast<MeshDev*>(static_cast<char*>(c->counters) + kMeshDev);
+}
+
+// graphs per context (least recently used evicted) and the largest mesh
+// captured: beyond it the launches are a small part of a step, and each graph
+// keeps its own buffers (about 0.8 GB at 128^3 in FP64)
+constexpr size_t kMaxGraphs = 8;
+constexpr int64_t kGraphMaxNodes = int64_t(1) << 21;
+
+std::string graph_key(osc_ctx* c, const int nn[3], double gamma, int cath, int calcb) {
+  struct {
+    int32_t n[3], prec, cath, calcb;
+    double gamma;
+  } k{{nn[0], nn[1], nn[2]}, c->prec, cath, calcb, gamma};
+  return std::string(reinterpret_cast<const char*>(&k), sizeof(k));
+}
+
+SolveGraph* step_graph(osc_ctx* c, const int nn[3], double gamma, int cath, int calcb) {
+  auto it = c->sgraphs.find(graph_key(c, nn, gamma, cath, calcb));
+  if (it == c->sgraphs.end()) return nullptr;
+  it->second->last = ++c->graph_tick;
+  return it->second.get();
+}
+
+// After an eager step of a new (shape, precision, solver, B, gamma): record
+// the same solve as graph nodes for the following steps (osc::launch adds
+// each kernel as a node while the recorder is set; nothing runs).  The eager
+// step has created the twiddle tables and left its temporaries in the pool,
+// so recording allocates (almost) nothing; a recording that fails leaves the
+// eager path in place for this shape.
+void note_step_shape(osc_ctx* c, const int nn[3], double gamma, int cath, int calcb,
+                     const osc_mesh& m) {
+  const int64_t nt = prod3(nn);
+  if (nt > kGraphMaxNodes || c->stream == nullptr) return;
+  const std::string key = graph_key(c, nn, gamma, cath, calcb);
+  if (c->sgraphs.count(key)) return;
+  if (c->sgraphs.size() >= kMaxGraphs) {
+    auto victim = c->sgraphs.begin();
+    for (auto it = c->sgraphs.begin(); it != c->sgraphs.end(); ++it)
+      if (it->second->last < victim->second->last) victim = it;
+    OSC_HIP(hipStreamSynchronize(c->stream));
+    c->sgraphs.erase(victim);
+  }
+  auto g = std::make_unique<SolveGraph>();
+  g->pool = &c->pool;
+  g->rho = DevBuf(c->pool, sizeof(double) * nt);
+  g->ef = DevBuf(c->pool, sizeof(double) * 3 * nt);
+  if (calcb) g->bf = DevBuf(c->pool, sizeof(double) * 3 * nt);
+  if (hipGraphCreate(&g->graph, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  GraphRecorder rec;
+  rec.graph = g->graph;
+  c->green_md = mesh_dev(c);
+  c->pool.retain = &g->kept;
+  t_recorder = &rec;
+  bool ok = true;
+  try {
+    space_charge_3d(c, m, g->rho.as<double>(), nullptr, cath, calcb, g->ef.as<double>(),
+                    calcb ? g->bf.as<double>() : nullptr);
+  } catch (const Error&) {
+    ok = false;
+  }
+  t_recorder = nullptr;
+  c->green_md = nullptr;
+  c->pool.retain = nullptr;
+  if (!ok || rec.failed || !rec.last ||
+      hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return;  // g's destructor gives its buffers back
+  }
+  g->last = ++c->graph_tick;
+  ++c->graph_builds;
+  c->sgraphs.emplace(key, std::move(g));
+}
+
+}  // namespace
+
+// finish_deposit_mesh: the host side of a deposit queued with defer_mesh
+static void finish_deposit_mesh(osc_ctx* c, bool resize_mesh, bool qa, osc_mesh* m) {
+  OSC_HIP(hipEventSynchronize(c->mesh_ev));
+  const MeshDev& h = *c->mesh_host;
+  if (resize_mesh)
+    for (int d = 0; d < 3; ++d) {
+      m->min[d] = h.mn[d];
+      m->max[d] = h.mx[d];
+      m->delta[d] = h.d[d];
+    }
+  if (qa) m->charge = h.charge;
+}
+
+// defer_mesh (with *deferred): a resized mesh or charge sum is not waited
+// for here; *deferred tells the caller to call finish_deposit_mesh
 static int deposit_staged(osc_ctx* c, Staging& st, int64_t n, const double* x, const double* y,
                           const double* z, const double* qa, const double* total_charge,
                           int resize_mesh, osc_mesh* m, double* rho, SortedParticles* keep,
-                          unsigned long long* hash = nullptr) {
+                          unsigned long long* hash = nullptr, bool* deferred = nullptr) {
   int nn[3];
   mesh_dims(m, nn);
   // The resized mesh (:176-189) and the sum of the charges (:244-245) are
@@ -279,17 +397,8 @@ static int deposit_staged(osc_ctx* c, Staging& st, int64_t n, const double* x, c
     a.delta[d] = m->delta[d];
   }
   if (qa || total_charge) deposit_cic(c, a, rho, keep);
-  if (readback) {
-    OSC_HIP(hipEventSynchronize(c->mesh_ev));
-    const MeshDev& h = *c->mesh_host;
-    if (resize_mesh)
-      for (int d = 0; d < 3; ++d) {
-        m->min[d] = h.mn[d];
-        m->max[d] = h.mx[d];
-        m->delta[d] = h.d[d];
-      }
-    if (qa) m->charge = h.charge;
-  }
+  if (deferred) *deferred = readback;
+  if (readback && !deferred) finish_deposit_mesh(c, resize_mesh, qa != nullptr, m);
   // upstream resizes the mesh before it finds no charge (:249-252)
   if (!qa && !total_charge)
     return fail(c, OSC_ERR_NO_CHARGE, "Error, must specify qa or total_charge");
@@ -358,9 +467,40 @@ OSC_API int osc_space_charge_step(osc_ctx* c, int64_t n, const double* xa, const
     double* Eo = st.out(E, size_t(3 * n));
     double* Bo = st.out(B, size_t(3 * n));
     SortedParticles sp;
-    int rc = deposit_staged(c, st, n, x, y, z, qa, total_charge, resize_mesh, m, r, &sp);
-    if (rc != OSC_OK) return rc;
-    space_charge_3d(c, *m, r, offset, at_cathode, calc_bfield, e, b);
+    // a mesh that follows the bunch is known only on the device until the
+    // deposit is done: its solve is replayed from a graph that builds the
+    // Green functions from the device mesh, with no host round trip.  A
+    // fixed mesh runs eagerly, its Green spectra cached for the next step
+    // of the same geometry.
+    const bool zero_off = !offset || (offset[0] == 0.0 && offset[1] == 0.0 && offset[2] == 0.0);
+    const bool graphable = c->graphs && !c->prof && zero_off && resize_mesh;
+    SolveGraph* g = graphable ? step_graph(c, nn, m->gamma, at_cathode, calc_bfield) : nullptr;
+    if (g) {
+      // the captured solve: deposit into its rho, the graph (reading the
+      // mesh on the device), the mesh for the host while it runs
+      bool deferred = false;
+      int rc = deposit_staged(c, st, n, x, y, z, qa, total_charge, resize_mesh, m,
+                              g->rho.as<double>(), &sp, nullptr, &deferred);
+      if (rc != OSC_OK) return rc;
+      OSC_HIP(hipGraphLaunch(g->exec, c->stream));
+      ++g->replays;
+      ++c->graph_replays;
+      if (deferred) finish_deposit_mesh(c, resize_mesh != 0, qa != nullptr, m);
+      OSC_HIP(hipMemcpyAsync(r, g->rho.p, sizeof(double) * nt, hipMemcpyDeviceToDevice, c->stream));
+      OSC_HIP(hipMemcpyAsync(e, g->ef.p, sizeof(double) * 3 * nt, hipMemcpyDeviceToDevice,
+                             c->stream));
+      if (b)
+        OSC_HIP(hipMemcpyAsync(b, g->bf.p, sizeof(double) * 3 * nt, hipMemcpyDeviceToDevice,
+                               c->stream));
+      // the gather reads the graph's fields (the copies run beside it in order)
+      e = g->ef.as<double>();
+      b = calc_bfield ? g->bf.as<double>() : nullptr;
+    } else {
+      int rc = deposit_staged(c, st, n, x, y, z, qa, total_charge, resize_mesh, m, r, &sp);
+      if (rc != OSC_OK) return rc;
+      space_charge_3d(c, *m, r, offset, at_cathode, calc_bfield, e, b);
+      if (graphable) note_step_shape(c, nn, m->gamma, at_cathode, calc_bfield, *m);
+    }
     if (Eo || Bo) {
       InterpArgs ia{};
       for (int d = 0; d < 3; ++d) {

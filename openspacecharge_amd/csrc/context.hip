@@ -22,6 +22,23 @@ static size_t bucket(size_t bytes) {
 
 void* DevPool::get(size_t bytes) {
   size_t bb = bucket(bytes);
+  if (retain) {
+    // while a graph is captured, its own freed temporaries are reused first:
+    // its launches run in stream order, as eager ones do
+    auto best = retain->end();
+    for (auto it = retain->begin(); it != retain->end(); ++it)
+      if (it->second >= bb && it->second <= 2 * bb &&
+          (best == retain->end() || it->second < best->second))
+        best = it;
+    if (best != retain->end()) {
+      void* p = best->first;
+      const size_t have = best->second;
+      retain->erase(best);
+      live_[p] = have;
+      live_bytes_ += have;
+      return p;
+    }
+  }
   // best fit among cached buffers of at most twice the request, so that
   // requests of varying sizes (a stream of bunches) reuse memory
   auto it = free_.lower_bound(bb);
@@ -57,8 +74,19 @@ void DevPool::put(void* p) {
   size_t bb = it->second;
   live_.erase(it);
   live_bytes_ -= bb;
+  if (retain) {
+    retain->emplace_back(p, bb);
+    return;
+  }
   free_.emplace(bb, p);
   cached_ += bb;
+}
+
+thread_local GraphRecorder* t_recorder = nullptr;
+
+void DevPool::adopt(void* p, size_t bytes) {
+  free_.emplace(bytes, p);
+  cached_ += bytes;
 }
 
 void DevPool::release_free() {
@@ -101,7 +129,10 @@ const cpx<T>* twiddles(osc_ctx* c, int L) {
   }
   void* d = nullptr;
   OSC_HIP(hipMalloc(&d, sizeof(cpx<T>) * L));
-  OSC_HIP(hipMemcpy(d, h.data(), sizeof(cpx<T>) * L, hipMemcpyHostToDevice));
+  // on the context's stream (a legacy-stream copy fails while another thread
+  // captures a stream), waited for before h goes away
+  OSC_HIP(hipMemcpyAsync(d, h.data(), sizeof(cpx<T>) * L, hipMemcpyHostToDevice, c->stream));
+  OSC_HIP(hipStreamSynchronize(c->stream));
   tab[L] = d;
   return static_cast<const cpx<T>*>(d);
 }

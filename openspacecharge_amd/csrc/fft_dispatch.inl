@@ -20,11 +20,11 @@ void axis_t(int dir, const cpx<T>* in, cpx<T>* out, const AxisIO& a, int g1, int
   constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
   dim3 g = grid3(cdiv(a.nb, BX), g1, g2);
   if (dir < 0 && a.mh > 0)
-    hipLaunchKernelGGL((k_axis<T, L, -1, true>), g, dim3(NT), 0, s, in, out, a, tw);
+    launch((k_axis<T, L, -1, true>), g, dim3(NT), 0, s, in, out, a, tw);
   else if (dir < 0)
-    hipLaunchKernelGGL((k_axis<T, L, -1, false>), g, dim3(NT), 0, s, in, out, a, tw);
+    launch((k_axis<T, L, -1, false>), g, dim3(NT), 0, s, in, out, a, tw);
   else
-    hipLaunchKernelGGL((k_axis<T, L, 1, false>), g, dim3(NT), 0, s, in, out, a, tw);
+    launch((k_axis<T, L, 1, false>), g, dim3(NT), 0, s, in, out, a, tw);
 }
 template <int L>
 void xr2c_t(const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw, hipStream_t s,
@@ -32,15 +32,15 @@ void xr2c_t(const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw, hip
   constexpr int BX = XGeo<T, L>::BX, NT = XGeo<T, L>::NT;
   dim3 g = grid3(cdiv(cdiv(a.nrows, 2), BX), 1, 1);
   if (cd)
-    hipLaunchKernelGGL((k_xr2c<T, L, true>), g, dim3(NT), 0, s, in, out, a, tw, *cd);
+    launch((k_xr2c<T, L, true>), g, dim3(NT), 0, s, in, out, a, tw, *cd);
   else
-    hipLaunchKernelGGL((k_xr2c<T, L, false>), g, dim3(NT), 0, s, in, out, a, tw, CornerDiff{});
+    launch((k_xr2c<T, L, false>), g, dim3(NT), 0, s, in, out, a, tw, CornerDiff{});
 }
 template <int L>
 void xc2r_t(const cpx<T>* in, const C2RIO<T>& a, int nout, const cpx<T>* tw, hipStream_t s) {
   constexpr int BX = XGeo<T, L>::BX, NT = XGeo<T, L>::NT;
   dim3 g = grid3(cdiv(cdiv(a.nrows, 2), BX), nout, 1);
-  hipLaunchKernelGGL((k_xc2r<T, L>), g, dim3(NT), 0, s, in, a, tw);
+  launch((k_xc2r<T, L>), g, dim3(NT), 0, s, in, a, tw);
 }
 // k_zconv's per-thread byte offsets (line b < BX, position t < TPL of a
 // column block) must fit in 32 bits
@@ -60,15 +60,15 @@ void zinv_t(const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, int Ly, int nout
   if (a.zreal && a.zin > 0) {
     OSC_REQUIRE(zconv_fits_t<L>(a), OSC_ERR_ARG, "fused z convolution: offsets exceed 32 bits");
     const int nxb = int(cdiv(a.nb, ZGeo<T, L>::BX));
-    hipLaunchKernelGGL((k_zconv<T, L>), grid3(int64_t(nxb) * Ly, 1, 1), dim3(ZGeo<T, L>::NT), 0,
+    launch((k_zconv<T, L>), grid3(int64_t(nxb) * Ly, 1, 1), dim3(ZGeo<T, L>::NT), 0,
                        s, rho, out, a, tw, nxb);
   } else if (a.zreal) {
     const int nxb = int(cdiv(a.nb, ZGeo<T, L>::BX));
     const int64_t units = int64_t(nxb) * Ly;
     const dim3 g1 = grid3(cdiv(units, 8) * 8 * nout, 1, 1);
-    hipLaunchKernelGGL((k_zinv_g<T, L>), g1, dim3(ZGeo<T, L>::NT), 0, s, rho, out, a, tw, nxb);
+    launch((k_zinv_g<T, L>), g1, dim3(ZGeo<T, L>::NT), 0, s, rho, out, a, tw, nxb);
   } else
-    hipLaunchKernelGGL((k_zinv<T, L>), g, dim3(NT), 0, s, rho, out, a, tw);
+    launch((k_zinv<T, L>), g, dim3(NT), 0, s, rho, out, a, tw);
 }
 }  // namespace
 
@@ -88,10 +88,10 @@ void launch_axis<T>(int L, int dir, const cpx<T>* in, cpx<T>* out, const AxisIO&
       GenericPlan pl = generic_plan(L);
       dim3 g = grid3(cdiv(a.nb, kGenBX), g1, g2);
       if (dir < 0)
-        hipLaunchKernelGGL((k_axis_gen<T, -1>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in,
+        launch((k_axis_gen<T, -1>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in,
                            out, a, tw, pl);
       else
-        hipLaunchKernelGGL((k_axis_gen<T, 1>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in,
+        launch((k_axis_gen<T, 1>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in,
                            out, a, tw, pl);
     }
   }
@@ -112,7 +112,7 @@ void launch_xr2c<T>(int L, const double* in, cpx<T>* out, const R2CIO& a, const 
     default: {
       GenericPlan pl = generic_plan(L);
       dim3 g = grid3(cdiv(cdiv(a.nrows, 2), kGenBX), 1, 1);
-      hipLaunchKernelGGL((k_xr2c_gen<T>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in, out,
+      launch((k_xr2c_gen<T>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in, out,
                          a, tw, pl);
     }
   }
@@ -150,7 +150,7 @@ void launch_xc2r<T>(int L, const cpx<T>* in, const C2RIO<T>& a, int nout, const 
     default: {
       GenericPlan pl = generic_plan(L);
       dim3 g = grid3(cdiv(cdiv(a.nrows, 2), kGenBX), nout, 1);
-      hipLaunchKernelGGL((k_xc2r_gen<T>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in, a,
+      launch((k_xc2r_gen<T>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, in, a,
                          tw, pl);
     }
   }
@@ -185,7 +185,7 @@ void launch_zinv<T>(int L, const cpx<T>* rho, cpx<T>* out, const ZinvIO<T>& a, i
                   "z-real Green arrays need a planned length");
       GenericPlan pl = generic_plan(L);
       dim3 g = grid3(cdiv(a.nb, kGenBX), Ly, nout);
-      hipLaunchKernelGGL((k_zinv_gen<T>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, rho, out,
+      launch((k_zinv_gen<T>), g, dim3(kGenBX * kGenTPL), gen_lds_bytes(L), s, rho, out,
                          a, tw, pl);
     }
   }

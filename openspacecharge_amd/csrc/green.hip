@@ -26,11 +26,31 @@ inline int64_t nblk(int64_t n, int t = kThreads) {
 // along `mod` the negated value is stored at L-1-i; there the mirror half
 // differs from a direct evaluation only by the rounding of the quotient
 // inside the last logarithm.
+//
+// With md set, the spacing comes from the mesh record on the device (a
+// captured solve replayed for meshes the host has not seen yet) and dx, dy,
+// dz, factor and the minima are formed here with the host's operations in
+// the host's order (build_offset_primitive); img selects the z offset of the
+// cathode image solve, 2 min(3) + (max(3) - min(3)) (open_spacecharge_mod.f90:472-476).
 __global__ void __launch_bounds__(64)
     k_prim_offset(double* __restrict__ F, int L0, int L1, int L2, double dx, double dy, double dz,
                   double umin, double vmin, double wmin, double factor, int icomp, int mev,
-                  int mod) {
+                  int mod, const MeshDev* __restrict__ md, double gamma, int img) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (md) {
+    dx = md->d[0];
+    dy = md->d[1];
+    dz = md->d[2] * gamma;
+    factor = (icomp == 1 || icomp == 2) ? gamma / (dx * dy * dz) : 1.0 / (dx * dy * dz);
+    umin = (0.5 - double(L0 / 2)) * dx;
+    vmin = (0.5 - double(L1 / 2)) * dy;
+    wmin = (0.5 - double(L2 / 2)) * dz;
+    umin = umin + 0.0;
+    vmin = vmin + 0.0;
+    double oz = 0.0;
+    if (img) oz = oz + 2 * md->mn[2] + (md->mx[2] - md->mn[2]);
+    wmin = wmin + oz * gamma;
+  }
   const bool m0 = mev == 0 || mod == 0, m1 = mev == 1 || mod == 1, m2 = mev == 2 || mod == 2;
   if (i >= (m0 ? L0 / 2 : L0)) return;
   const double w = m2 ? (double(k) + (0.5 - double(L2 / 2))) * dz : double(k) * dz + wmin;
@@ -247,8 +267,15 @@ static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F, bo
   if (mev == 0 || mod == 0) gp.x = unsigned((s.L[0] / 2 + 63) / 64);
   if (mev == 1 || mod == 1) gp.y = unsigned(s.L[1] / 2);
   if (mev == 2 || mod == 2) gp.z = unsigned(s.L[2] / 2);
-  hipLaunchKernelGGL(k_prim_offset, gp, dim3(64), 0, c->stream, F, s.L[0], s.L[1], s.L[2], dx, dy,
-                     dz, umin, vmin, wmin, factor, s.icomp, mev, mod);
+  // a solve being captured for replay takes the spacing from the device mesh
+  // record; only the zero offset of the free solve and the image offset of
+  // the cathode solve occur there
+  const MeshDev* md = c->green_md;
+  OSC_REQUIRE(!md || (s.offset[0] == 0.0 && s.offset[1] == 0.0), OSC_ERR_ARG,
+              "device-side Green parameters: unsupported offset");
+  launch(k_prim_offset, gp, dim3(64), 0, c->stream, F, s.L[0], s.L[1], s.L[2], dx, dy,
+                     dz, umin, vmin, wmin, factor, s.icomp, mev, mod, md, s.gamma,
+                     int(md && s.out_shift_z != 0));
   OSC_HIP(hipGetLastError());
 }
 
@@ -260,7 +287,7 @@ void build_green_real(osc_ctx* c, const GreenSpec& s, double* g, bool fast) {
     DevBuf F(c->pool, sizeof(double) * N);
     build_offset_primitive(c, s, F.as<double>(), fast);
     const dim3 g3(unsigned((s.L[0] + 63) / 64), unsigned(s.L[1]), unsigned(s.L[2]));
-    hipLaunchKernelGGL(k_diff_offset, g3, dim3(64), 0, c->stream, F.as<const double>(), g, s.L[0],
+    launch(k_diff_offset, g3, dim3(64), 0, c->stream, F.as<const double>(), g, s.L[0],
                        s.L[1], s.L[2], s.out_shift_z, s.zero_raw);
     OSC_HIP(hipGetLastError());
   } else if (s.kind == GK_PIPE) {
@@ -271,14 +298,14 @@ void build_green_real(osc_ctx* c, const GreenSpec& s, double* g, bool fast) {
     double* cv = cu + 5 * s.L[0];
     double* zf = cv + 5 * s.L[1];
     double* kap = zf + 25 * s.L[2];
-    hipLaunchKernelGGL(k_pipe_tables, dim3((lmax + 63) / 64), dim3(64), 0, c->stream, s, si, sj, cu,
+    launch(k_pipe_tables, dim3((lmax + 63) / 64), dim3(64), 0, c->stream, s, si, sj, cu,
                        cv, zf, kap);
     OSC_HIP(hipGetLastError());
     const dim3 g3(unsigned((s.L[0] + 63) / 64), unsigned(s.L[1]), unsigned(s.L[2]));
-    hipLaunchKernelGGL(k_green_pipe, g3, dim3(64), 0, c->stream, g, s, cu, cv, zf, kap);
+    launch(k_green_pipe, g3, dim3(64), 0, c->stream, g, s, cu, cv, zf, kap);
     OSC_HIP(hipGetLastError());
   } else {
-    hipLaunchKernelGGL(k_green_legacy, dim3(nblk(N)), dim3(kThreads), 0, c->stream, g, s);
+    launch(k_green_legacy, dim3(nblk(N)), dim3(kThreads), 0, c->stream, g, s);
     OSC_HIP(hipGetLastError());
   }
 }
@@ -344,7 +371,7 @@ template void spectrum_forward_full<float>(osc_ctx*, const double*, cpx<float>*,
 
 void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_t n) {
   ProfScope ps(c, ST_GREEN_FFT);
-  hipLaunchKernelGGL(k_convert, dim3(nblk(n)), dim3(kThreads), 0, c->stream, in, out, n);
+  launch(k_convert, dim3(nblk(n)), dim3(kThreads), 0, c->stream, in, out, n);
   OSC_HIP(hipGetLastError());
 }
 
@@ -361,7 +388,10 @@ GreenRef green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) {
   std::string key(reinterpret_cast<const char*>(&s), sizeof(s));
   GreenCache& gc = c->gcache;
   GreenCache::Entry* hit = nullptr;
-  if (gc.enable) {
+  // a captured solve builds its spectra on every replay (the spacing is only
+  // known on the device then): no cache
+  const bool use_cache = gc.enable && !c->green_md;
+  if (use_cache) {
     auto it = gc.map.find(key);
     if (it != gc.map.end()) {
       hit = it->second.get();
@@ -401,7 +431,7 @@ GreenRef green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) {
   }
   g.reset();
   ref.g = sp.p;
-  if (!gc.enable) {
+  if (!use_cache) {
     keepalive = std::move(sp);
     return ref;
   }

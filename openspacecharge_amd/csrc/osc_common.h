@@ -8,12 +8,63 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
 #include "../../include/opensc_hip.h"
 
 namespace osc {
+
+// ---------------------------------------------------------------------------
+// kernel launches, recordable into a HIP graph
+// ---------------------------------------------------------------------------
+// While a host thread records a solve (t_recorder set), launch() adds each
+// kernel as a node of the recorder's graph, after the previous one (the
+// stream order of the eager launches), instead of launching it.  Explicit
+// nodes need no stream capture, which would make other threads' legacy-stream
+// copies fail while it lasts.
+struct GraphRecorder {
+  hipGraph_t graph = nullptr;
+  hipGraphNode_t last = nullptr;
+  bool failed = false;
+};
+extern thread_local GraphRecorder* t_recorder;
+
+template <typename... KArgs, typename... Args>
+inline void launch(void (*k)(KArgs...), dim3 grid, dim3 block, size_t shm, hipStream_t s,
+                   Args&&... args) {
+  static_assert(sizeof...(KArgs) == sizeof...(Args), "kernel argument count");
+  // the arguments converted to the kernel's parameter types, one address each
+  std::tuple<std::decay_t<KArgs>...> vals(std::forward<Args>(args)...);
+  void* ptrs[sizeof...(KArgs) > 0 ? sizeof...(KArgs) : 1] = {};
+  std::apply(
+      [&](auto&... v) {
+        size_t i = 0;
+        ((ptrs[i++] = static_cast<void*>(&v)), ...);
+      },
+      vals);
+  if (GraphRecorder* r = t_recorder) {
+    hipKernelNodeParams kp{};
+    kp.func = reinterpret_cast<void*>(k);
+    kp.gridDim = grid;
+    kp.blockDim = block;
+    kp.sharedMemBytes = unsigned(shm);
+    kp.kernelParams = ptrs;
+    kp.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    if (hipGraphAddKernelNode(&node, r->graph, r->last ? &r->last : nullptr, r->last ? 1 : 0,
+                              &kp) != hipSuccess) {
+      (void)hipGetLastError();
+      r->failed = true;
+      return;
+    }
+    r->last = node;
+    return;
+  }
+  (void)hipLaunchKernel(reinterpret_cast<const void*>(k), grid, block, ptrs, shm, s);
+}
 
 // ---------------------------------------------------------------------------
 // complex helpers (interleaved re/im, same layout as Fortran complex)
@@ -88,6 +139,11 @@ class DevPool {
   void release_all();
   size_t bytes_cached() const { return cached_; }
   size_t bytes_live() const { return live_bytes_; }
+  // while set, buffers given back are moved here instead of the free list
+  // (the buffers a captured HIP graph keeps using), and get() takes from
+  // here first; adopt() returns one to the free list
+  std::vector<std::pair<void*, size_t>>* retain = nullptr;
+  void adopt(void* p, size_t bytes);
   ~DevPool() { release_all(); }
 
  private:

@@ -160,6 +160,26 @@ static_assert(kHashHost + 16 <= kNoallocHost && kNoallocHost + 4 <= kPinnedBytes
 
 namespace osc {
 class HostRing;
+
+// The grid part of a fused step (Green build, spectral solve, B) for one
+// (shape, precision, solver, B, gamma), captured once as a HIP graph and
+// replayed for every later mesh of that shape: the Green primitive reads the
+// spacing from the device mesh record, so the host needs no mesh first.  The
+// graph owns its buffers (rho in, fields out, every temporary of the solve).
+struct SolveGraph {
+  DevPool* pool = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  DevBuf rho, ef, bf;
+  std::vector<std::pair<void*, size_t>> kept;  // temporaries the graph uses
+  uint64_t last = 0;
+  int64_t replays = 0;
+  ~SolveGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& pb : kept) pool->adopt(pb.first, pb.second);
+  }
+};
 }
 
 struct osc_ctx {
@@ -189,6 +209,13 @@ struct osc_ctx {
   std::vector<hipEvent_t> prof_free;
   double prof_ms[osc::ST_COUNT] = {0};
   int64_t prof_n[osc::ST_COUNT] = {0};
+  // captured solves of the fused step (osc_ctx_set_graphs), and the device
+  // mesh record the Green primitive reads while one is being captured
+  bool graphs = false;
+  std::map<std::string, std::unique_ptr<osc::SolveGraph>> sgraphs;
+  uint64_t graph_tick = 0;
+  int64_t graph_builds = 0, graph_replays = 0;
+  const osc::MeshDev* green_md = nullptr;
 };
 
 namespace osc {

@@ -13,7 +13,7 @@
 // accounting: the budget (free HBM at creation less an eighth, or the
 // caller's limit) covers the estimated bytes of the running requests plus
 // what every worker's context holds between requests (Green caches, the pipe
-// Green functions, pooled buffers); each worker's Green cache is capped at
+// Green functions, captured solves, pooled buffers); each worker's Green cache is capped at
 // budget / (8 nworkers) and its pooled buffers are given back when it holds
 // more than budget / (2 nworkers) after a request.  A request larger than the
 // budget runs alone.
@@ -239,7 +239,9 @@ void osc_server::run_worker(int w) {
     // what the context keeps between requests: Green cache, pipe Green
     // functions and pooled buffers; pooled buffers beyond the share go back
     if (int64_t(c->pool.bytes_cached() + c->pool.bytes_live()) > share) c->pool.release_free();
-    const int64_t keep = int64_t(c->pool.bytes_cached() + c->pool.bytes_live());
+    int64_t keep = int64_t(c->pool.bytes_cached() + c->pool.bytes_live());
+    for (auto& kv : c->sgraphs)  // the temporaries of captured solves
+      for (auto& pb : kv.second->kept) keep += int64_t(pb.second);
     {
       std::lock_guard<std::mutex> lk(mu);
       if (d.status != OSC_OK) last_error = c->last_error;

@@ -76,7 +76,7 @@ __global__ void k_flipz(double* rho, int nx, int ny, int nz) {
 
 void fill_f64(osc_ctx* c, double* p, int64_t n, double v) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(kThreads), 0, c->stream, p, n, v);
+  launch(k_fill, dim3(nblk(n)), dim3(kThreads), 0, c->stream, p, n, v);
   OSC_HIP(hipGetLastError());
 }
 
@@ -84,7 +84,7 @@ void fd_gradient(osc_ctx* c, const double* phi, const int n[3], const double del
                  double* ef) {
   const int64_t nt = int64_t(n[0]) * n[1] * n[2];
   ProfScope ps(c, ST_FD_B);
-  hipLaunchKernelGGL(k_fd, dim3(nblk(nt)), dim3(kThreads), 0, c->stream, phi, n[0], n[1], n[2],
+  launch(k_fd, dim3(nblk(nt)), dim3(kThreads), 0, c->stream, phi, n[0], n[1], n[2],
                      delta[0], delta[1], delta[2], gam, ef);
   OSC_HIP(hipGetLastError());
 }
@@ -92,7 +92,7 @@ void fd_gradient(osc_ctx* c, const double* phi, const int n[3], const double del
 void bfield_core(osc_ctx* c, const double* ef, int64_t n, double gam, double* bf) {
   const double beta0 = sqrt(1.0 - 1.0 / (gam * gam));
   ProfScope ps(c, ST_FD_B);
-  hipLaunchKernelGGL(k_bcore, dim3(nblk(n)), dim3(kThreads), 0, c->stream, ef, n, beta0,
+  launch(k_bcore, dim3(nblk(n)), dim3(kThreads), 0, c->stream, ef, n, beta0,
                      kClightDp, bf);
   OSC_HIP(hipGetLastError());
 }
@@ -100,7 +100,7 @@ void bfield_core(osc_ctx* c, const double* ef, int64_t n, double gam, double* bf
 void flip_z(osc_ctx* c, double* rho, const int n[3]) {
   const int64_t nt = int64_t(n[0]) * n[1] * (n[2] / 2);
   if (nt == 0) return;
-  hipLaunchKernelGGL(k_flipz, dim3(nblk(nt)), dim3(kThreads), 0, c->stream, rho, n[0], n[1], n[2]);
+  launch(k_flipz, dim3(nblk(nt)), dim3(kThreads), 0, c->stream, rho, n[0], n[1], n[2]);
   OSC_HIP(hipGetLastError());
 }
 
