@@ -354,7 +354,8 @@ static void finish_deposit_mesh(osc_ctx* c, bool resize_mesh, bool qa, osc_mesh*
 static int deposit_staged(osc_ctx* c, Staging& st, int64_t n, const double* x, const double* y,
                           const double* z, const double* qa, const double* total_charge,
                           int resize_mesh, osc_mesh* m, double* rho, SortedParticles* keep,
-                          unsigned long long* hash = nullptr, bool* deferred = nullptr) {
+                          unsigned long long* hash = nullptr, bool* deferred = nullptr,
+                          bool f32 = false) {
   int nn[3];
   mesh_dims(m, nn);
   // The resized mesh (:176-189) and the sum of the charges (:244-245) are
@@ -367,6 +368,7 @@ static int deposit_staged(osc_ctx* c, Staging& st, int64_t n, const double* x, c
   a.y = y;
   a.z = z;
   a.hash = hash;
+  a.f32 = f32;
   bool readback = false;
   if (resize_mesh) {
     const int cells[3] = {m->nhi[0] - m->nlo[0], m->nhi[1] - m->nlo[1], m->nhi[2] - m->nlo[2]};
@@ -480,7 +482,8 @@ OSC_API int osc_space_charge_step(osc_ctx* c, int64_t n, const double* xa, const
       // mesh on the device), the mesh for the host while it runs
       bool deferred = false;
       int rc = deposit_staged(c, st, n, x, y, z, qa, total_charge, resize_mesh, m,
-                              g->rho.as<double>(), &sp, nullptr, &deferred);
+                              g->rho.as<double>(), &sp, nullptr, &deferred,
+                              c->prec == OSC_PREC_FP32);
       if (rc != OSC_OK) return rc;
       OSC_HIP(hipGraphLaunch(g->exec, c->stream));
       ++g->replays;
@@ -496,7 +499,10 @@ OSC_API int osc_space_charge_step(osc_ctx* c, int64_t n, const double* xa, const
       e = g->ef.as<double>();
       b = calc_bfield ? g->bf.as<double>() : nullptr;
     } else {
-      int rc = deposit_staged(c, st, n, x, y, z, qa, total_charge, resize_mesh, m, r, &sp);
+      // an FP32 context deposits in FP32 here (the step's own gather uses the
+      // weights); the standalone deposit_particles stays FP64
+      int rc = deposit_staged(c, st, n, x, y, z, qa, total_charge, resize_mesh, m, r, &sp,
+                              nullptr, nullptr, c->prec == OSC_PREC_FP32);
       if (rc != OSC_OK) return rc;
       space_charge_3d(c, *m, r, offset, at_cathode, calc_bfield, e, b);
       if (graphable) note_step_shape(c, nn, m->gamma, at_cathode, calc_bfield, *m);
@@ -510,7 +516,7 @@ OSC_API int osc_space_charge_step(osc_ctx* c, int64_t n, const double* xa, const
         ia.mn[d] = m->min[d];
         ia.delta[d] = m->delta[d];
       }
-      interpolate_sorted(c, ia, e, b, sp, Eo, Bo, nullptr);
+      interpolate_sorted(c, ia, e, b, sp, Eo, Bo, nullptr, nullptr, x, y, z);
     }
     if (n_dropped) *n_dropped = read_counter(c, 0);
     if (n_clamped) *n_clamped = read_counter(c, 2);

@@ -150,8 +150,8 @@ static_assert((RS_TILE / RS_W) % 64 == 0, "each wave of the scatter ranks whole 
 // step 1: cell keys
 // ---------------------------------------------------------------------------
 struct KeyParams {
-  double mn[3], dinv[3], base[3];
-  int nlo[3], nn[3], nhi[3];
+  double mn[3], dinv[3], base[3], d[3];
+  int nlo[3], nn[3], nhi[3], wofs[3];
   uint32_t sentinel;
   int clamp;  // 0: deposit rule (off-mesh particles get the sentinel); 1: gather rule
   const MeshDev* md;  // if set: mn and dinv come from here (a mesh resized on the device)
@@ -250,6 +250,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 // and leaves the tile's count of each first-pass digit (key & mask0) in
 // hist[d * nb + b], the layout of k_rs_hist.  With hpart, the workgroup's
 // partial content hash of (x, y, z) goes to hpart[b].
+// F32 (FP32 deposition): the record is instead the 16-byte (ab, de, gh, q)
+// in FP32 -- the CIC weights formed in FP64 by the upstream formula
+// (open_spacecharge_mod.f90:262-264) from the FP64 cell index, then rounded
+// -- stored by each lane directly (consecutive lanes, consecutive records).
+template <bool F32>
 __global__ void __launch_bounds__(KEY_T)
     k_cic_keys(const double* __restrict__ x, const double* __restrict__ y,
                const double* __restrict__ z, const double* __restrict__ q, int64_t n, KeyParams p,
@@ -266,6 +271,7 @@ __global__ void __launch_bounds__(KEY_T)
     for (int d = 0; d < 3; ++d) {
       p.mn[d] = p.md->mn[d];
       p.dinv[d] = p.md->dinv[d];
+      p.d[d] = p.md->d[d];
     }
   __syncthreads();
   if (p.clear)
@@ -306,7 +312,17 @@ __global__ void __launch_bounds__(KEY_T)
       keys[i] = key;
       atomicAdd(&h[key & mask0], 1u);
       mine += ok ? 0u : 1u;
+      if (F32) {
+        float4 r = make_float4(0.f, 0.f, 0.f, float(qi));
+        if (ok) {
+          r.x = float(((p.mn[0] - xi) + double(i0 + p.wofs[0]) * p.d[0]) * p.dinv[0]);
+          r.y = float(((p.mn[1] - yi) + double(j0 + p.wofs[1]) * p.d[1]) * p.dinv[1]);
+          r.z = float(((p.mn[2] - zi) + double(k0 + p.wofs[2]) * p.d[2]) * p.dinv[2]);
+        }
+        reinterpret_cast<float4*>(rec)[i] = r;
+      }
     }
+    if (F32) continue;  // uniform: the 16-byte records are stored above
     w[lane * 5] = xi;
     w[lane * 5 + 1] = yi;
     w[lane * 5 + 2] = zi;
@@ -563,14 +579,26 @@ __device__ __forceinline__ double dpp_f64(double v) {
   const uint32_t hi = dpp_u32<CTRL, ROWMASK>(0u, uint32_t(b >> 32));
   return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
 }
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __uint_as_float(dpp_u32<CTRL, ROWMASK>(0u, __float_as_uint(v)));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_any(double v) {
+  return dpp_f64<CTRL, ROWMASK>(v);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_any(float v) {
+  return dpp_f32<CTRL, ROWMASK>(v);
+}
 // one step of the segmented scan: add the source lane's partial when it holds
 // the same key (keys are sorted, so equal keys are one contiguous run)
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ void seg_step(uint32_t k, double (&t)[8]) {
+template <int CTRL, int ROWMASK, typename A>
+__device__ __forceinline__ void seg_step(uint32_t k, A (&t)[8]) {
   const uint32_t kd = dpp_u32<CTRL, ROWMASK>(0xFFFFFFFFu, k);
-  double o[8];
+  A o[8];
 #pragma unroll
-  for (int c = 0; c < 8; ++c) o[c] = dpp_f64<CTRL, ROWMASK>(t[c]);
+  for (int c = 0; c < 8; ++c) o[c] = dpp_any<CTRL, ROWMASK>(t[c]);
   if (kd == k)
 #pragma unroll
     for (int c = 0; c < 8; ++c) t[c] = o[c] + t[c];
@@ -684,6 +712,66 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// FP32 deposition: the same run decomposition over the 16-byte weight
+// records (ab, de, gh, q) of k_cic_keys<true>, the 8 terms formed and summed
+// in FP32 in the upstream product order; cell sums, head and tail in FP32.
+// With KEEP the records go to srec in cell order for the gather.
+template <bool KEEP>
+__global__ void __launch_bounds__(kThreads)
+    k_seg_reduce32(const uint32_t* __restrict__ ks, const uint32_t* __restrict__ sv,
+                   const float4* __restrict__ rec, float4* __restrict__ srec, int64_t n,
+                   const uint32_t* __restrict__ ndrop, float q_each, int hasq,
+                   float* __restrict__ S, uint8_t* __restrict__ occ, float* __restrict__ head,
+                   float* __restrict__ tail, int64_t ncell) {
+  const int64_t nv = n - int64_t(*ndrop);
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t base = int64_t(blockIdx.x) * blockDim.x; base < n; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    const int64_t w = i >> 6;
+    const bool valid = i < nv;
+    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < n) r = rec[sv[i]];
+    if (KEEP && i < n) srec[i] = r;
+    const uint32_t k = valid ? ks[i] : 0xFFFFFFFFu;
+    float t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (valid) {
+      const float ab = r.x, de = r.y, gh = r.z, q = hasq ? r.w : q_each;
+      t[0] = ab * de * gh * q;
+      t[2] = ab * (1.f - de) * gh * q;
+      t[6] = ab * (1.f - de) * (1.f - gh) * q;
+      t[4] = ab * de * (1.f - gh) * q;
+      t[5] = (1.f - ab) * de * (1.f - gh) * q;
+      t[7] = (1.f - ab) * (1.f - de) * (1.f - gh) * q;
+      t[3] = (1.f - ab) * (1.f - de) * gh * q;
+      t[1] = (1.f - ab) * de * gh * q;
+    }
+    seg_step<kDppRowShr + 1, 0xF>(k, t);
+    seg_step<kDppRowShr + 2, 0xF>(k, t);
+    seg_step<kDppRowShr + 4, 0xF>(k, t);
+    seg_step<kDppRowShr + 8, 0xF>(k, t);
+    seg_step<kDppRowBcast15, 0xA>(k, t);
+    seg_step<kDppRowBcast31, 0xC>(k, t);
+    const uint32_t kn = __shfl_down(k, 1, 64);
+    const uint32_t kfirst = __shfl(k, 0, 64);
+    if (!valid || (lane < 63 && kn == k)) continue;  // not the last lane of its run
+    const int64_t w0 = w << 6;
+    const bool started = kfirst != k || w0 == 0 || ks[w0 - 1] != k;
+    const bool ends = lane < 63 || i + 1 >= nv || ks[i + 1] != k;
+    if (started && ends) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) S[c * ncell + k] = t[c];
+      occ[k] = 1;
+    } else if (ends) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) head[w * 8 + c] = t[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) tail[w * 8 + c] = t[c];
+    }
+  }
+}
+
 // Wave holding the last element of the run of key k that continues past
 // the end of wave w (ks sorted, nv valid keys): the first wave ww > w whose
 // last element is not k, or the last wave.  Galloping over wave ends, then
@@ -714,10 +802,11 @@ __device__ __forceinline__ int64_t run_end_wave(const uint32_t* __restrict__ ks,
 // One thread per wave whose last run continues into the next wave and began
 // in this wave: completes that run.  Runs spanning more than kSegSerial waves
 // are listed for k_seg_heavy.
+template <typename A>
 __global__ void k_seg_fix(const uint32_t* __restrict__ ks, int64_t n,
                           const uint32_t* __restrict__ ndrop, int64_t ncell,
-                          const double* __restrict__ head, const double* __restrict__ tail,
-                          double* __restrict__ S, uint8_t* __restrict__ occ,
+                          const A* __restrict__ head, const A* __restrict__ tail,
+                          A* __restrict__ S, uint8_t* __restrict__ occ,
                           uint32_t* __restrict__ heavy, uint32_t* __restrict__ nheavy) {
   const int64_t nv = n - int64_t(*ndrop);
   const int64_t nw = (nv + 63) >> 6;
@@ -730,7 +819,7 @@ __global__ void k_seg_fix(const uint32_t* __restrict__ ks, int64_t n,
     if (ks[w << 6] == k && w > 0 && ks[(w << 6) - 1] == k) continue;  // began earlier
     // the middle waves are summed in order as their ends are checked; past
     // kSegSerial of them the run goes to k_seg_heavy
-    double acc[8];
+    A acc[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) acc[c] = tail[w * 8 + c];
     int64_t ww = w + 1;
@@ -758,19 +847,20 @@ __global__ void k_seg_fix(const uint32_t* __restrict__ ks, int64_t n,
 
 // One workgroup per long run: thread t sums the middle-wave partials
 // w+1+t, w+1+t+kThreads, ... in order, then a fixed LDS tree.
+template <typename A>
 __global__ void __launch_bounds__(kThreads)
     k_seg_heavy(const uint32_t* __restrict__ ks, int64_t n, const uint32_t* __restrict__ ndrop,
-                int64_t ncell, const double* __restrict__ head, const double* __restrict__ tail,
-                double* __restrict__ S, uint8_t* __restrict__ occ,
+                int64_t ncell, const A* __restrict__ head, const A* __restrict__ tail,
+                A* __restrict__ S, uint8_t* __restrict__ occ,
                 const uint32_t* __restrict__ heavy, const uint32_t* __restrict__ nheavy) {
-  __shared__ double red[8][kThreads];
+  __shared__ A red[8][kThreads];
   const int64_t nv = n - int64_t(*ndrop);
   const uint32_t nh = *nheavy;
   for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
     const int64_t w = heavy[h];
     const uint32_t k = ks[(w << 6) + 63];
     const int64_t w1 = run_end_wave(ks, nv, w, k);
-    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    A s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int64_t ww = w + 1 + threadIdx.x; ww < w1; ww += kThreads)
 #pragma unroll
       for (int c = 0; c < 8; ++c) s[c] = s[c] + tail[ww * 8 + c];
@@ -804,7 +894,8 @@ __global__ void __launch_bounds__(kThreads)
 // doubles never yields -0 in round-to-nearest, so adding a selected +0 leaves
 // it unchanged: the node values are those of the conditional sum.  Cell
 // counts are below 2^31 (deposit_cic), so the index arithmetic is 32-bit.
-__global__ void k_node(const double* __restrict__ S, const uint8_t* __restrict__ occ, int nx,
+template <typename A>
+__global__ void k_node(const A* __restrict__ S, const uint8_t* __restrict__ occ, int nx,
                        int ny, int nz, int scale_after, double scale, double* __restrict__ rho) {
   const uint32_t ncell = uint32_t(nx) * uint32_t(ny) * uint32_t(nz);
   const uint32_t sxy = uint32_t(nx) * uint32_t(ny);
@@ -812,19 +903,19 @@ __global__ void k_node(const double* __restrict__ S, const uint8_t* __restrict__
        c += gridDim.x * blockDim.x) {
     const uint32_t k = c / sxy, r = c - k * sxy, j = r / uint32_t(nx), i = r - j * uint32_t(nx);
     uint8_t f[8];
-    double v[8];
+    A v[8];
 #pragma unroll
     for (int cc = 0; cc < 8; ++cc) {
       const int di = cc & 1, dj = (cc >> 1) & 1, dk = cc >> 2;
       const bool in = i >= uint32_t(di) && j >= uint32_t(dj) && k >= uint32_t(dk);
       const uint32_t cell = c - uint32_t(di) - uint32_t(dj) * uint32_t(nx) - uint32_t(dk) * sxy;
       f[cc] = in ? occ[cell] : uint8_t(0);
-      v[cc] = in ? S[size_t(cc) * ncell + cell] : 0.0;
+      v[cc] = in ? S[size_t(cc) * ncell + cell] : A(0);
     }
-    double acc = 0.0;
+    A acc = A(0);
 #pragma unroll
-    for (int cc = 0; cc < 8; ++cc) acc = acc + (f[cc] ? v[cc] : 0.0);
-    rho[c] = scale_after ? acc * scale : acc;
+    for (int cc = 0; cc < 8; ++cc) acc = acc + (f[cc] ? v[cc] : A(0));
+    rho[c] = scale_after ? double(acc) * scale : double(acc);
   }
 }
 
@@ -998,10 +1089,14 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   uint32_t* cnt = static_cast<uint32_t*>(c->counters);
   fill_u32(c, cnt, 4, 0u);
   const int64_t n = a.n;
+  // FP32 deposition (the charge-mode deposit of an FP32 context): 16-byte
+  // weight records, FP32 sums
+  const bool f32 = a.f32 && !a.harness;
+  const size_t rb = f32 ? sizeof(float4) : sizeof(double4);
   DevBuf ka(c->pool, sizeof(uint32_t) * (n + 1)), va(c->pool, sizeof(uint32_t) * (n + 1));
   DevBuf kb(c->pool, sizeof(uint32_t) * (n + 1)), vb(c->pool, sizeof(uint32_t) * (n + 1));
-  DevBuf rec(c->pool, sizeof(double4) * (n + 1));
-  DevBuf srec(c->pool, keep ? sizeof(double4) * (n + 1) : 0);
+  DevBuf rec(c->pool, rb * (n + 1));
+  DevBuf srec(c->pool, keep ? rb * (n + 1) : 0);
   KeyParams kp{};
   CellParams cp{};
   for (int d = 0; d < 3; ++d) {
@@ -1010,6 +1105,8 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
     kp.base[d] = a.harness ? double(a.nlo[d]) : 1.0;
     kp.nlo[d] = a.nlo[d];
     kp.nn[d] = a.nn[d];
+    kp.d[d] = a.delta[d];
+    kp.wofs[d] = a.harness ? 1 : a.nlo[d];
     cp.mn[d] = a.mn[d];
     cp.d[d] = a.delta[d];
     cp.dinv[d] = kp.dinv[d];
@@ -1041,9 +1138,14 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   if (n > 0) {
     ProfScope ps(c, ST_DEP_KEYS);
     DevBuf hpart(c->pool, a.hash ? sizeof(uint64_t) * sp.nb : 0);
-    hipLaunchKernelGGL(k_cic_keys, dim3(sp.nb), dim3(KEY_T), 0, c->stream, a.x, a.y, a.z, a.q, n, kp,
-                       ka.as<uint32_t>(), rec.as<double4>(), cnt, sp.mask(0), hist.as<uint32_t>(),
-                       sp.nb, hpart.as<uint64_t>());
+    if (f32)
+      hipLaunchKernelGGL(k_cic_keys<true>, dim3(sp.nb), dim3(KEY_T), 0, c->stream, a.x, a.y, a.z,
+                         a.q, n, kp, ka.as<uint32_t>(), rec.as<double4>(), cnt, sp.mask(0),
+                         hist.as<uint32_t>(), sp.nb, hpart.as<uint64_t>());
+    else
+      hipLaunchKernelGGL(k_cic_keys<false>, dim3(sp.nb), dim3(KEY_T), 0, c->stream, a.x, a.y, a.z,
+                         a.q, n, kp, ka.as<uint32_t>(), rec.as<double4>(), cnt, sp.mask(0),
+                         hist.as<uint32_t>(), sp.nb, hpart.as<uint64_t>());
     OSC_HIP(hipGetLastError());
     if (a.hash) {
       hipLaunchKernelGGL(k_hash_sum, dim3(1), dim3(kThreads), 0, c->stream,
@@ -1061,11 +1163,22 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   // corner sums of every cell (zero where no particle), wave partials of runs
   // crossing wave boundaries, list of long runs
   const int64_t nw = (n + 63) / 64 + 1;
-  DevBuf S(c->pool, sizeof(double) * 8 * ncell), head(c->pool, sizeof(double) * 8 * nw),
-      tail(c->pool, sizeof(double) * 8 * nw), heavy(c->pool, sizeof(uint32_t) * nw);
+  const size_t ab = f32 ? sizeof(float) : sizeof(double);
+  DevBuf S(c->pool, ab * 8 * ncell), head(c->pool, ab * 8 * nw), tail(c->pool, ab * 8 * nw),
+      heavy(c->pool, sizeof(uint32_t) * nw);
   if (n > 0) {
     ProfScope ps_cells(c, ST_DEP_CELLS);
-    if (keep)
+    if (f32 && keep)
+      hipLaunchKernelGGL(k_seg_reduce32<true>, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk,
+                         sv, rec.as<const float4>(), srec.as<float4>(), n, cnt, float(a.q_each),
+                         int(a.q != nullptr), S.as<float>(), occ.as<uint8_t>(), head.as<float>(),
+                         tail.as<float>(), ncell);
+    else if (f32)
+      hipLaunchKernelGGL(k_seg_reduce32<false>, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk,
+                         sv, rec.as<const float4>(), nullptr, n, cnt, float(a.q_each),
+                         int(a.q != nullptr), S.as<float>(), occ.as<uint8_t>(), head.as<float>(),
+                         tail.as<float>(), ncell);
+    else if (keep)
       hipLaunchKernelGGL(k_seg_reduce<true>, dim3(nblk(n)), dim3(kThreads), 0, c->stream, sk, sv,
                          rec.as<const double4>(), srec.as<double4>(), n, cnt,
                          cp, S.as<double>(), occ.as<uint8_t>(), head.as<double>(),
@@ -1077,26 +1190,43 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
     OSC_HIP(hipGetLastError());
     ps_cells.stop();
     ProfScope pf(c, ST_DEP_FIX);
-    hipLaunchKernelGGL(k_seg_fix, dim3(nblk(nw)), dim3(kThreads), 0, c->stream, sk, n, cnt, ncell,
-                       head.as<const double>(), tail.as<const double>(), S.as<double>(),
-                       occ.as<uint8_t>(), heavy.as<uint32_t>(), cnt + 1);
-    OSC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_seg_heavy, dim3(256), dim3(kThreads), 0, c->stream, sk, n, cnt, ncell,
-                       head.as<const double>(), tail.as<const double>(), S.as<double>(),
-                       occ.as<uint8_t>(), heavy.as<const uint32_t>(), cnt + 1);
+    if (f32) {
+      hipLaunchKernelGGL(k_seg_fix<float>, dim3(nblk(nw)), dim3(kThreads), 0, c->stream, sk, n,
+                         cnt, ncell, head.as<const float>(), tail.as<const float>(),
+                         S.as<float>(), occ.as<uint8_t>(), heavy.as<uint32_t>(), cnt + 1);
+      OSC_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_seg_heavy<float>, dim3(256), dim3(kThreads), 0, c->stream, sk, n, cnt,
+                         ncell, head.as<const float>(), tail.as<const float>(), S.as<float>(),
+                         occ.as<uint8_t>(), heavy.as<const uint32_t>(), cnt + 1);
+    } else {
+      hipLaunchKernelGGL(k_seg_fix<double>, dim3(nblk(nw)), dim3(kThreads), 0, c->stream, sk, n,
+                         cnt, ncell, head.as<const double>(), tail.as<const double>(),
+                         S.as<double>(), occ.as<uint8_t>(), heavy.as<uint32_t>(), cnt + 1);
+      OSC_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_seg_heavy<double>, dim3(256), dim3(kThreads), 0, c->stream, sk, n, cnt,
+                         ncell, head.as<const double>(), tail.as<const double>(),
+                         S.as<double>(), occ.as<uint8_t>(), heavy.as<const uint32_t>(), cnt + 1);
+    }
     OSC_HIP(hipGetLastError());
   }
   {
     ProfScope ps(c, ST_DEP_NODES);
-    hipLaunchKernelGGL(k_node, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,
-                       S.as<const double>(), occ.as<const uint8_t>(), nx, ny, nz, a.harness,
-                       a.scale_after, rho);
+    if (f32)
+      hipLaunchKernelGGL(k_node<float>, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,
+                         S.as<const float>(), occ.as<const uint8_t>(), nx, ny, nz, a.harness,
+                         a.scale_after, rho);
+    else
+      hipLaunchKernelGGL(k_node<double>, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,
+                         S.as<const double>(), occ.as<const uint8_t>(), nx, ny, nz, a.harness,
+                         a.scale_after, rho);
     OSC_HIP(hipGetLastError());
   }
   if (keep) {
     keep->n = n;
     keep->sv = std::move(in_b ? vb : va);
     keep->srec = std::move(srec);
+    keep->f32 = f32;
+    if (f32) keep->sk = std::move(in_b ? kb : ka);
   }
 }
 
@@ -1125,7 +1255,7 @@ void sort_for_gather(osc_ctx* c, const InterpArgs& a, int64_t n, const double* x
   const SortPlan sp = sort_plan(nbits, n);
   DevBuf hist(c->pool, sizeof(uint32_t) * 256 * size_t(sp.nb + 1));
   if (n > 0) {
-    hipLaunchKernelGGL(k_cic_keys, dim3(sp.nb), dim3(KEY_T), 0, c->stream, x, y, z,
+    hipLaunchKernelGGL(k_cic_keys<false>, dim3(sp.nb), dim3(KEY_T), 0, c->stream, x, y, z,
                        static_cast<const double*>(nullptr), n, kp, ka.as<uint32_t>(),
                        rec.as<double4>(), cnt, sp.mask(0), hist.as<uint32_t>(), sp.nb,
                        static_cast<uint64_t*>(nullptr));

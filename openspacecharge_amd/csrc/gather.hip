@@ -323,6 +323,208 @@ __global__ void __launch_bounds__(kThreads)
   if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 particle path (the fused step of an FP32 context)
+// ---------------------------------------------------------------------------
+// node records in FP32: (E0, E1, E2, B0, B1, B2, 0, 0), 32 bytes per node
+__global__ void __launch_bounds__(kThreads)
+    k_pack_nodes32(const double* __restrict__ ef, const double* __restrict__ bf, int64_t nt,
+                   float* __restrict__ rec) {
+  __shared__ float stage[kThreads / 64][64 * 9];
+  const int lane = threadIdx.x & 63;
+  float* w = stage[threadIdx.x >> 6];
+  for (int64_t i0 = blockIdx.x * int64_t(blockDim.x) + (threadIdx.x & ~63); i0 < nt;
+       i0 += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = i0 + lane;
+    const int nn = nt - i0 < 64 ? int(nt - i0) : 64;
+    if (i < nt) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        w[lane * 9 + c] = ef ? float(ef[c * nt + i]) : 0.f;
+        w[lane * 9 + 3 + c] = bf ? float(bf[c * nt + i]) : 0.f;
+      }
+      w[lane * 9 + 6] = 0.f;
+      w[lane * 9 + 7] = 0.f;
+    }
+    wave_lds_sync();
+    float4* o = reinterpret_cast<float4*>(rec + 8 * i0);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = q * 64 + lane;  // 16-byte element: record e / 2, half e % 2
+      if ((e >> 1) < nn) {
+        const float* src = w + (e >> 1) * 9 + 4 * (e & 1);
+        o[e] = make_float4(src[0], src[1], src[2], src[3]);
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
+template <typename A, typename F>
+__device__ __forceinline__ A wsum8_any(F&& N, A ab, A de, A gh) {
+  const A one = A(1);
+  return N(0, 0) * ab * de * gh + N(1, 0) * ab * (one - de) * gh +
+         N(3, 0) * ab * (one - de) * (one - gh) + N(2, 0) * ab * de * (one - gh) +
+         N(2, 1) * (one - ab) * de * (one - gh) + N(3, 1) * (one - ab) * (one - de) * (one - gh) +
+         N(1, 1) * (one - ab) * (one - de) * gh + N(0, 1) * (one - ab) * de * gh;
+}
+
+// The cell-ordered gather of k_interp_cells in FP32: the cell is the deposit's
+// sorted key and the weights its FP32 records (the deposit and interpolate
+// rules agree for every particle on the mesh); a dropped particle (sentinel
+// key, off the mesh) takes the clamp rule from its FP64 coordinates.  Node
+// rows staged in LDS as in k_interp_cells; results (E, B, clamped, 0) as
+// FP32 32-byte records at the particle's original position.
+__global__ void __launch_bounds__(kThreads)
+    k_interp_cells32(const float* __restrict__ R, int wantE, int wantB, int64_t n,
+                     const float4* __restrict__ srec, const uint32_t* __restrict__ sk,
+                     const uint32_t* __restrict__ sv, const double* __restrict__ x,
+                     const double* __restrict__ y, const double* __restrict__ z, IP p,
+                     float* __restrict__ out, uint32_t* __restrict__ nclamp) {
+  __shared__ uint32_t cnt;
+  __shared__ float stage[kThreads / 64][64 * kRecPad];
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  uint32_t mine = 0;
+  const int lane = threadIdx.x & 63;
+  float* st = stage[threadIdx.x >> 6];
+  const int64_t sy = p.nn[0], sz = int64_t(p.nn[0]) * p.nn[1];
+  const uint32_t ncell = uint32_t(p.nn[0]) * uint32_t(p.nn[1]) * uint32_t(p.nn[2]);
+  for (int64_t wb = blockIdx.x * int64_t(blockDim.x) + (threadIdx.x & ~63); wb < n;
+       wb += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t s = wb + lane;
+    float v[6] = {0, 0, 0, 0, 0, 0};
+    bool fl = false;
+    float ab = 0, de = 0, gh = 0;
+    int64_t b = 0;
+    if (s < n) {
+      const uint32_t key = sk[s];
+      if (key < ncell) {
+        const float4 r = srec[s];
+        ab = r.x;
+        de = r.y;
+        gh = r.z;
+        b = int64_t(key);
+      } else {
+        const uint32_t i = sv[s];
+        const double xi = x[i], yi = y[i], zi = z[i];
+        const int ip = clamp_idx(xi, p.mn[0], p.hinv[0], p.nhi[0], fl);
+        const int jp = clamp_idx(yi, p.mn[1], p.hinv[1], p.nhi[1], fl);
+        const int kp = clamp_idx(zi, p.mn[2], p.hinv[2], p.nhi[2], fl);
+        ab = float(((p.mn[0] - xi) + ip * p.delta[0]) * p.hinv[0]);
+        de = float(((p.mn[1] - yi) + jp * p.delta[1]) * p.hinv[1]);
+        gh = float(((p.mn[2] - zi) + kp * p.delta[2]) * p.hinv[2]);
+        int i0 = ip - p.nlo[0], j0 = jp - p.nlo[1], k0 = kp - p.nlo[2];
+        i0 = i0 < 0 ? 0 : (i0 > p.nn[0] - 2 ? p.nn[0] - 2 : i0);
+        j0 = j0 < 0 ? 0 : (j0 > p.nn[1] - 2 ? p.nn[1] - 2 : j0);
+        k0 = k0 < 0 ? 0 : (k0 > p.nn[2] - 2 ? p.nn[2] - 2 : k0);
+        b = i0 + j0 * sy + k0 * sz;
+      }
+      mine += fl ? 1u : 0u;
+    }
+    const long long b0 = __shfl(static_cast<long long>(b), 0);
+    long long lo = s < n ? static_cast<long long>(b) : b0, hi = lo;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const long long ol = __shfl_xor(lo, off), oh = __shfl_xor(hi, off);
+      lo = ol < lo ? ol : lo;
+      hi = oh > hi ? oh : hi;
+    }
+    if (hi - lo <= kNodeWin - 2) {
+      const int r = lane / kNodeWin, q = lane % kNodeWin;
+      if (q <= int(hi - lo) + 1) {
+        const int64_t node = lo + q + ((r & 1) ? sy : 0) + ((r & 2) ? sz : 0);
+        const float4* src = reinterpret_cast<const float4*>(R + 8 * node);
+        const float4 e0 = src[0], e1 = src[1];
+        st[lane * 6 + 0] = e0.x;
+        st[lane * 6 + 1] = e0.y;
+        st[lane * 6 + 2] = e0.z;
+        st[lane * 6 + 3] = e0.w;
+        st[lane * 6 + 4] = e1.x;
+        st[lane * 6 + 5] = e1.y;
+      }
+      wave_lds_sync();
+      if (s < n) {
+        const int q0 = int(b - lo);
+        for (int c = wantE ? 0 : 3; c < (wantB ? 6 : 3); ++c)
+          v[c] = wsum8_any<float>(
+              [&](int rr, int d) { return st[((rr * kNodeWin) + q0 + d) * 6 + c]; }, ab, de, gh);
+      }
+      wave_lds_sync();
+    } else if (s < n) {
+      const int64_t i1 = 8, j1 = 8 * sy, k1 = 8 * sz;
+      for (int c = wantE ? 0 : 3; c < (wantB ? 6 : 3); ++c) {
+        const int64_t o = 8 * b + c;
+        auto N = [&](int rr, int d) {
+          return R[o + ((rr & 1) ? j1 : 0) + ((rr & 2) ? k1 : 0) + (d ? i1 : 0)];
+        };
+        v[c] = wsum8_any<float>(N, ab, de, gh);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) st[lane * kRecPad + c] = v[c];
+    st[lane * kRecPad + 6] = fl ? 1.f : 0.f;
+    st[lane * kRecPad + 7] = 0.f;
+    const uint32_t dst = s < n ? sv[s] : 0u;
+    wave_lds_sync();
+    // round j: lanes 2k, 2k+1 store the two halves of record 32 j + k
+    const int part = lane & 1;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int r = 32 * j + (lane >> 1);
+      const uint32_t d = uint32_t(__shfl(int(dst), r));
+      if (wb + r < n) {
+        const float* sr = st + r * kRecPad + 4 * part;
+        reinterpret_cast<float4*>(out + 8 * int64_t(d))[part] = make_float4(sr[0], sr[1], sr[2], sr[3]);
+      }
+    }
+    wave_lds_sync();
+  }
+  if (mine) atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
+}
+
+// the FP32 32-byte records back to particle order, E and B as FP64
+__global__ void __launch_bounds__(kThreads)
+    k_unpack32(const float* __restrict__ out, int64_t n, double* __restrict__ E,
+               double* __restrict__ B, int8_t* __restrict__ clamped) {
+  __shared__ float stage[kThreads / 64][64 * 8];
+  const int lane = threadIdx.x & 63;
+  float* w = stage[threadIdx.x >> 6];
+  const float4* in4 = reinterpret_cast<const float4*>(out);
+  for (int64_t i0 = blockIdx.x * int64_t(blockDim.x) + (threadIdx.x & ~63); i0 < n;
+       i0 += int64_t(gridDim.x) * blockDim.x) {
+    const int nrec = n - i0 < 64 ? int(n - i0) : 64;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = lane + 64 * q;  // 16-byte element of the chunk, record e / 2
+      if ((e >> 1) < nrec) {
+        const float4 v = in4[2 * i0 + e];
+        w[4 * e] = v.x;
+        w[4 * e + 1] = v.y;
+        w[4 * e + 2] = v.z;
+        w[4 * e + 3] = v.w;
+      }
+    }
+    wave_lds_sync();
+    if (E)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int e = lane + 64 * k;
+        if (e / 3 < nrec) E[3 * i0 + e] = double(w[(e / 3) * 8 + e % 3]);
+      }
+    if (B)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int e = lane + 64 * k;
+        if (e / 3 < nrec) B[3 * i0 + e] = double(w[(e / 3) * 8 + 3 + e % 3]);
+      }
+    if (clamped && lane < nrec) clamped[i0 + lane] = w[lane * 8 + 6] != 0.f ? 1 : 0;
+    wave_lds_sync();
+  }
+}
+
 // original order: the 64-byte records out8[i] = (E0, E1, E2, B0, B1, B2,
 // flag, pad) into E(3,n), B(3,n) and the clamp flags.  Each wave moves 64
 // records through LDS: read as 4 contiguous KB, written as contiguous runs of
@@ -377,7 +579,8 @@ void pack_nodes(osc_ctx* c, const InterpArgs& a, const double* ef, const double*
 
 void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
                         const SortedParticles& sp, double* E, double* B, int8_t* clamped,
-                        const double* packed) {
+                        const double* packed, const double* x, const double* y,
+                        const double* z) {
   IP p{};
   for (int d = 0; d < 3; ++d) {
     p.mn[d] = a.mn[d];
@@ -391,6 +594,36 @@ void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const
   const int64_t nt = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
   uint32_t* cnt = static_cast<uint32_t*>(c->counters) + 2;
   fill_u32(c, cnt, 1, 0u);
+  if (sp.f32) {
+    // FP32: 32-byte node and result records, the deposit's cells and weights
+    OSC_REQUIRE(x && y && z && !packed, OSC_ERR_ARG, "FP32 gather: coordinates needed");
+    DevBuf rec32(c->pool, sizeof(float) * 8 * nt);
+    {
+      ProfScope pp(c, ST_GATHER_PACK);
+      const int64_t nbp = std::min<int64_t>((nt + kThreads - 1) / kThreads, 65536 * 8);
+      hipLaunchKernelGGL(k_pack_nodes32, dim3(unsigned(nbp)), dim3(kThreads), 0, c->stream, ef,
+                         bf, nt, rec32.as<float>());
+      OSC_HIP(hipGetLastError());
+    }
+    const int64_t n = sp.n;
+    if (n <= 0) return;
+    DevBuf out32(c->pool, sizeof(float) * 8 * n);
+    const int64_t nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
+    {
+      ProfScope ps(c, ST_GATHER);
+      hipLaunchKernelGGL(k_interp_cells32, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
+                         rec32.as<const float>(), int(E && ef), int(B && bf), n,
+                         sp.srec.as<const float4>(), sp.sk.as<const uint32_t>(),
+                         sp.sv.as<const uint32_t>(), x, y, z, p, out32.as<float>(), cnt);
+      OSC_HIP(hipGetLastError());
+    }
+    ProfScope po(c, ST_GATHER_OUT);
+    hipLaunchKernelGGL(k_unpack32, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
+                       out32.as<const float>(), n, (E && ef) ? E : nullptr,
+                       (B && bf) ? B : nullptr, clamped);
+    OSC_HIP(hipGetLastError());
+    return;
+  }
   DevBuf rec(c->pool, packed ? 0 : sizeof(double) * 8 * nt);
   if (!packed) {
     pack_nodes(c, a, ef, bf, rec.as<double>());

@@ -129,7 +129,10 @@ struct ProfRec {
 struct SortedParticles {
   int64_t n = 0;
   DevBuf sv;    // original particle index of each sorted slot (u32)
-  DevBuf srec;  // 32-byte records (x, y, z, q) in sorted order (double4)
+  DevBuf srec;  // 32-byte records (x, y, z, q) in sorted order (double4), or with
+                // f32 16-byte records (ab, de, gh, q) of the FP32 deposit (float4)
+  bool f32 = false;
+  DevBuf sk;    // f32: the sorted cell keys (u32; the sentinel for dropped particles)
 };
 
 // a gather of at least this many points runs in cell order
@@ -397,6 +400,7 @@ struct DepositArgs {
   double scale_after;   // harness chrgpermacro
   unsigned long long* hash;  // if set: receives the content hash of (x, y, z)
   const MeshDev* md;  // if set: the kernels read mn and delta from here
+  int f32;  // FP32 deposition: weights rounded to FP32 per particle, sums in FP32
 };
 // rho (nx,ny,nz) on device; the dropped count is left in counter 0
 // (read_counter(c, 0)); keep (optional) receives the sorted particles
@@ -440,9 +444,12 @@ void interpolate_packed(osc_ctx* c, const InterpArgs& a, const double* ef, const
 int noalloc_setup(int device, std::string* err);
 // same, for the particles of the preceding deposit in their cell-sorted order;
 // packed: the node records of (ef, bf) already queued by pack_nodes, or null
+// (sp.f32: the FP32 path, which reads the coordinates x, y, z of dropped
+// particles)
 void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf,
                         const SortedParticles& sp, double* E, double* B, int8_t* clamped,
-                        const double* packed = nullptr);
+                        const double* packed = nullptr, const double* x = nullptr,
+                        const double* y = nullptr, const double* z = nullptr);
 // queues the 64-byte node records (E, B) of the mesh into rec (8 doubles per node)
 void pack_nodes(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf, double* rec);
 // stable sort of n points by the mesh cell their gather reads (the clamped
