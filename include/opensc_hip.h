@@ -112,6 +112,16 @@ int osc_fft_length_supported(int n);
 /* 1 if this length has a compile-time radix plan (LDS-tiled multi-line passes);
  * other supported lengths use the runtime-radix engine, one line per workgroup */
 int osc_fft_length_planned(int n);
+/* Tile width (lines per workgroup) of the strided FFT passes, chosen per
+ * (precision, length, pass kind, line count) by an online tuner: the first
+ * calls of each run the candidate widths (half, default, double) in turn,
+ * timed by events, then the fastest is kept; results do not depend on the
+ * width.  mode 1: tune (default); 0: default width; 2, 3, 4: force the half,
+ * default or double width.  OSC_FFT_TUNE_FILE (environment) keeps the
+ * decisions across processes.  Process-wide. */
+int osc_fft_tuning(int mode);
+/* lengths/kinds decided, still measuring */
+int osc_fft_tune_stats(int64_t* out2);
 
 /* ---- deposition --------------------------------------------------------------
  * deposit_particles (open_spacecharge_mod.f90:156-288).

@@ -96,6 +96,8 @@ SIGNATURES = {
     "osc_abi_version": (C.c_int, []),
     "osc_fft_length_supported": (C.c_int, [C.c_int]),
     "osc_fft_length_planned": (C.c_int, [C.c_int]),
+    "osc_fft_tuning": (C.c_int, [C.c_int]),
+    "osc_fft_tune_stats": (C.c_int, [C.POINTER(_i64)]),
     "osc_ctx_create": (C.c_int, [C.c_int, C.POINTER(_ctxp)]),
     "osc_ctx_destroy": (C.c_int, [_ctxp]),
     "osc_ctx_set_stream": (C.c_int, [_ctxp, _vp]),

@@ -67,6 +67,18 @@ OSC_API int osc_fft_length_supported(int n) { return fft_len_supported(n) ? 1 : 
 
 OSC_API int osc_fft_length_planned(int n) { return fft_has_plan(n) ? 1 : 0; }
 
+OSC_API int osc_fft_tuning(int mode) {
+  if (mode < 0 || mode > 4) return OSC_ERR_ARG;
+  fft_tune_mode(mode);
+  return OSC_OK;
+}
+
+OSC_API int osc_fft_tune_stats(int64_t* out2) {
+  if (!out2) return OSC_ERR_ARG;
+  fft_tune_stats(&out2[0], &out2[1]);
+  return OSC_OK;
+}
+
 OSC_API int osc_ctx_create(int device, osc_ctx** out) {
   if (!out) return OSC_ERR_ARG;
   *out = nullptr;

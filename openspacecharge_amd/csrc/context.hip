@@ -2,6 +2,7 @@
 // FFT length support.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <condition_variable>
 #include <thread>
 
@@ -83,6 +84,186 @@ void DevPool::put(void* p) {
 }
 
 thread_local GraphRecorder* t_recorder = nullptr;
+
+// ---------------------------------------------------------------------------
+// FFT tile-width tuner (FftTune)
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kTuneSamples = 2;
+struct TuneEntry {
+  int prec, L, kind, bucket;
+  int decided = -1;
+  bool distinct[3] = {false, true, false};
+  float best[3] = {1e30f, 1e30f, 1e30f};
+  int count[3] = {0, 0, 0};
+  int pending_n[3] = {0, 0, 0};
+  struct Pending {
+    int var;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+};
+std::mutex g_tune_mu;
+std::vector<TuneEntry> g_tune;  // few keys: linear search
+int g_tune_mode = 1;
+bool g_tune_loaded = false;
+
+int tune_bucket(int64_t lines) {
+  int b = 0;
+  while ((int64_t(1) << (b + 1)) <= lines) ++b;
+  return b;
+}
+
+void tune_load_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char* path = std::getenv("OSC_FFT_TUNE_FILE");
+  if (!path) return;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return;
+  int prec, L, kind, bucket, var;
+  while (std::fscanf(f, "%d %d %d %d %d", &prec, &L, &kind, &bucket, &var) == 5) {
+    if (var < 0 || var > 2) continue;
+    TuneEntry e;
+    e.prec = prec;
+    e.L = L;
+    e.kind = kind;
+    e.bucket = bucket;
+    e.decided = var;
+    g_tune.push_back(e);
+  }
+  std::fclose(f);
+}
+
+void tune_save(const TuneEntry& e) {
+  const char* path = std::getenv("OSC_FFT_TUNE_FILE");
+  if (!path) return;
+  FILE* f = std::fopen(path, "a");
+  if (!f) return;
+  std::fprintf(f, "%d %d %d %d %d\n", e.prec, e.L, e.kind, e.bucket, e.decided);
+  std::fclose(f);
+}
+
+// reads the completed timings of e without waiting; decides when it can
+void tune_poll_locked(TuneEntry& e) {
+  for (size_t i = 0; i < e.pending.size();) {
+    auto& p = e.pending[i];
+    if (hipEventQuery(p.b) != hipSuccess) {
+      (void)hipGetLastError();
+      ++i;
+      continue;
+    }
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      e.best[p.var] = std::min(e.best[p.var], ms);
+      ++e.count[p.var];
+    } else {
+      (void)hipGetLastError();
+    }
+    --e.pending_n[p.var];
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+    e.pending.erase(e.pending.begin() + long(i));
+  }
+  bool all = true;
+  for (int v = 0; v < 3; ++v) all = all && (!e.distinct[v] || e.count[v] >= kTuneSamples);
+  if (!all) return;
+  int best = 1;
+  for (int v = 0; v < 3; ++v)
+    if (e.distinct[v] && e.best[v] < e.best[best]) best = v;
+  e.decided = best;
+  tune_save(e);
+}
+}  // namespace
+
+FftTune::FftTune(int prec, int L, int kind, int64_t lines, const bool distinct[3], hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (g_tune_mode == 0) return;
+  if (g_tune_mode >= 2) {
+    const int v = g_tune_mode - 2;
+    var_ = distinct[v] ? v : 1;
+    return;
+  }
+  tune_load_locked();
+  const int bucket = tune_bucket(lines);
+  int k = -1;
+  for (size_t i = 0; i < g_tune.size(); ++i) {
+    const TuneEntry& e = g_tune[i];
+    if (e.prec == prec && e.L == L && e.kind == kind && e.bucket == bucket) k = int(i);
+  }
+  if (k < 0) {
+    TuneEntry e;
+    e.prec = prec;
+    e.L = L;
+    e.kind = kind;
+    e.bucket = bucket;
+    for (int v = 0; v < 3; ++v) e.distinct[v] = distinct[v];
+    g_tune.push_back(e);
+    k = int(g_tune.size()) - 1;
+  }
+  TuneEntry& e = g_tune[size_t(k)];
+  if (e.decided >= 0) {
+    var_ = e.decided;
+    return;
+  }
+  tune_poll_locked(e);
+  if (e.decided >= 0) {
+    var_ = e.decided;
+    return;
+  }
+  // a graph being recorded cannot hold timing events: the default width
+  if (t_recorder) return;
+  int pick = -1;
+  for (int v = 0; v < 3; ++v) {
+    if (!e.distinct[v] || e.count[v] + e.pending_n[v] >= kTuneSamples) continue;
+    if (pick < 0 || e.count[v] + e.pending_n[v] < e.count[pick] + e.pending_n[pick]) pick = v;
+  }
+  if (pick < 0) {  // every sample is in flight: the default width meanwhile
+    return;
+  }
+  if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess ||
+      hipEventRecord(a_, s) != hipSuccess) {
+    (void)hipGetLastError();
+    if (a_) (void)hipEventDestroy(a_);
+    if (b_) (void)hipEventDestroy(b_);
+    a_ = b_ = nullptr;
+    return;
+  }
+  var_ = pick;
+  key_ = k;
+  ++e.pending_n[pick];
+}
+
+void FftTune::done(hipStream_t s) {
+  if (key_ < 0) return;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  TuneEntry& e = g_tune[size_t(key_)];
+  if (hipEventRecord(b_, s) != hipSuccess) {
+    (void)hipGetLastError();
+    --e.pending_n[var_];
+    (void)hipEventDestroy(a_);
+    (void)hipEventDestroy(b_);
+    return;
+  }
+  e.pending.push_back({var_, a_, b_});
+}
+
+void fft_tune_mode(int mode) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_mode = mode;
+}
+
+void fft_tune_stats(int64_t* decided, int64_t* measuring) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  int64_t d = 0, m = 0;
+  for (auto& e : g_tune) {
+    if (e.decided < 0) tune_poll_locked(e);
+    if (e.decided >= 0) ++d;
+    else ++m;
+  }
+  if (decided) *decided = d;
+  if (measuring) *measuring = m;
+}
 
 void DevPool::adopt(void* p, size_t bytes) {
   free_.emplace(bytes, p);

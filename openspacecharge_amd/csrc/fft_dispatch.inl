@@ -14,17 +14,38 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 size_t gen_lds_bytes(int L) { return size_t(LdsPad<T>::size(2 * kGenBX * L)) * sizeof(cpx<T>); }
 
+template <int L, int V>
+void axis_tv(int dir, const cpx<T>* in, cpx<T>* out, const AxisIO& a, int g1, int g2,
+             const cpx<T>* tw, hipStream_t s) {
+  constexpr int BX = AxisTiles<T, L>::bx(V), NT = BX * Geo<T, L>::TPL;
+  dim3 g = grid3(cdiv(a.nb, BX), g1, g2);
+  if (dir < 0 && a.mh > 0)
+    launch((k_axis<T, L, -1, true, BX>), g, dim3(NT), 0, s, in, out, a, tw);
+  else if (dir < 0)
+    launch((k_axis<T, L, -1, false, BX>), g, dim3(NT), 0, s, in, out, a, tw);
+  else
+    launch((k_axis<T, L, 1, false, BX>), g, dim3(NT), 0, s, in, out, a, tw);
+}
+// the tile width the tuner picked for this pass (fft_tune_*)
 template <int L>
 void axis_t(int dir, const cpx<T>* in, cpx<T>* out, const AxisIO& a, int g1, int g2,
             const cpx<T>* tw, hipStream_t s) {
-  constexpr int BX = Geo<T, L>::BX, NT = BX * Geo<T, L>::TPL;
-  dim3 g = grid3(cdiv(a.nb, BX), g1, g2);
-  if (dir < 0 && a.mh > 0)
-    launch((k_axis<T, L, -1, true>), g, dim3(NT), 0, s, in, out, a, tw);
-  else if (dir < 0)
-    launch((k_axis<T, L, -1, false>), g, dim3(NT), 0, s, in, out, a, tw);
-  else
-    launch((k_axis<T, L, 1, false>), g, dim3(NT), 0, s, in, out, a, tw);
+  const int kind = dir > 0 ? 2 : (a.mh > 0 ? 1 : 0);
+  const int64_t lines = int64_t(a.nb) * g1 * g2;
+  bool distinct[3];
+  for (int v = 0; v < 3; ++v) distinct[v] = v == 1 || AxisTiles<T, L>::bx(v) != AxisTiles<T, L>::bx(1);
+  FftTune tune(int(sizeof(T)), L, kind, lines, distinct, s);
+  switch (tune.variant()) {
+    case 0:
+      axis_tv<L, 0>(dir, in, out, a, g1, g2, tw, s);
+      break;
+    case 2:
+      axis_tv<L, 2>(dir, in, out, a, g1, g2, tw, s);
+      break;
+    default:
+      axis_tv<L, 1>(dir, in, out, a, g1, g2, tw, s);
+  }
+  tune.done(s);
 }
 template <int L>
 void xr2c_t(const double* in, cpx<T>* out, const R2CIO& a, const cpx<T>* tw, hipStream_t s,

@@ -125,6 +125,22 @@ struct Geo {
   static constexpr int NT = BX * TPL;
 };
 
+// Tile widths (lines per workgroup) a strided pass can run with: half, the
+// default and twice Geo's, where the threads (<= 1024) and LDS (<= 64 KB)
+// allow, else the default.  The width changes only how many lines share a
+// workgroup, not the arithmetic of a line: results are bit-identical.
+template <typename T, int L>
+struct AxisTiles {
+  static constexpr int B0 = Geo<T, L>::BX, TPL = Geo<T, L>::TPL;
+  static constexpr bool fits(int bx) {
+    return bx >= 1 && bx * TPL <= 1024 &&
+           LdsPad<T>::size(bx * L) * (int)sizeof(cpx<T>) <= 65536;
+  }
+  static constexpr int bx(int v) {
+    return v == 0 ? (fits(B0 / 2) ? B0 / 2 : B0) : (v == 2 ? (fits(2 * B0) ? 2 * B0 : B0) : B0);
+  }
+};
+
 // x-pass geometry (contiguous rows, so no cache-line constraint on the
 // number of lines): half the lines of Geo per workgroup, half its LDS
 template <typename T, int L>
@@ -146,10 +162,9 @@ __device__ __forceinline__ cpx<T> ldc(const cpx<T>* p) {
 // strided line FFT (y or z axis, or x with s_p = 1)
 // ---------------------------------------------------------------------------
 // MIR: the input lines are stored as halves (AxisIO::mh)
-template <typename T, int L, int DIR, bool MIR>
-__global__ void __launch_bounds__((Geo<T, L>::NT))
+template <typename T, int L, int DIR, bool MIR, int BX = Geo<T, L>::BX>
+__global__ void __launch_bounds__((BX * Geo<T, L>::TPL))
     k_axis(const cpx<T>* __restrict__ in, cpx<T>* out, AxisIO a, const cpx<T>* __restrict__ tw) {
-  constexpr int BX = Geo<T, L>::BX;
   using F = FftTile<T, L, DIR, BX, false>;
   __shared__ cpx<T> lds[F::LDS];
   F f;

@@ -12,6 +12,29 @@ namespace osc {
       X(864) X(972) X(1024)
 
 bool fft_has_plan(int L);
+
+// Online tuner of the strided passes' tile width (AxisTiles).  For each
+// (precision, L, pass kind, log2 of the line count) the first calls run the
+// distinct candidate widths in turn, each bracketed by two events on the
+// launch stream; once every candidate has kTuneSamples completed timings
+// (read without waiting, hipEventQuery) the fastest is used from then on.
+// Widths do not change results, so the calls that measure are real work.
+// fft_tune_mode: 1 tune (default), 0 the default width, 2 + v width v.
+// With OSC_FFT_TUNE_FILE set, decisions are read from and appended to it.
+class FftTune {
+ public:
+  FftTune(int prec, int L, int kind, int64_t lines, const bool distinct[3], hipStream_t s);
+  int variant() const { return var_; }
+  void done(hipStream_t s);
+
+ private:
+  int key_ = -1;
+  int var_ = 1;
+  hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+void fft_tune_mode(int mode);
+// keys decided, keys still measuring
+void fft_tune_stats(int64_t* decided, int64_t* measuring);
 // true when every prime factor of L is <= 23 and the generic engine fits LDS
 bool fft_len_supported(int L);
 GenericPlan generic_plan(int L);
