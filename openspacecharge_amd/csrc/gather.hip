@@ -373,21 +373,26 @@ __device__ __forceinline__ A wsum8_any(F&& N, A ab, A de, A gh) {
 // sorted key and the weights its FP32 records (the deposit and interpolate
 // rules agree for every particle on the mesh); a dropped particle (sentinel
 // key, off the mesh) takes the clamp rule from its FP64 coordinates.  Node
-// rows staged in LDS as in k_interp_cells; results (E, B, clamped, 0) as
-// FP32 32-byte records at the particle's original position.
+// rows staged in LDS as in k_interp_cells; the results (E, B, clamped, 0)
+// leave as the 64-byte FP64 records of k_interp_cells at the particle's
+// original position (measured: 32-byte FP32 records at random places,
+// 0.34 ms for 10^7 points, are slower to store than 64-byte ones), for
+// k_unpack.
 __global__ void __launch_bounds__(kThreads)
     k_interp_cells32(const float* __restrict__ R, int wantE, int wantB, int64_t n,
                      const float4* __restrict__ srec, const uint32_t* __restrict__ sk,
                      const uint32_t* __restrict__ sv, const double* __restrict__ x,
                      const double* __restrict__ y, const double* __restrict__ z, IP p,
-                     float* __restrict__ out, uint32_t* __restrict__ nclamp) {
+                     double* __restrict__ out8, uint32_t* __restrict__ nclamp) {
   __shared__ uint32_t cnt;
-  __shared__ float stage[kThreads / 64][64 * kRecPad];
+  __shared__ float stage[kThreads / 64][64 * 6];
+  __shared__ double ostage[kThreads / 64][64 * kRecPad];
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   uint32_t mine = 0;
   const int lane = threadIdx.x & 63;
   float* st = stage[threadIdx.x >> 6];
+  double* ot = ostage[threadIdx.x >> 6];
   const int64_t sy = p.nn[0], sz = int64_t(p.nn[0]) * p.nn[1];
   const uint32_t ncell = uint32_t(p.nn[0]) * uint32_t(p.nn[1]) * uint32_t(p.nn[2]);
   for (int64_t wb = blockIdx.x * int64_t(blockDim.x) + (threadIdx.x & ~63); wb < n;
@@ -445,15 +450,20 @@ __global__ void __launch_bounds__(kThreads)
       }
       wave_lds_sync();
       if (s < n) {
+        // compile-time component indices keep v in registers
         const int q0 = int(b - lo);
-        for (int c = wantE ? 0 : 3; c < (wantB ? 6 : 3); ++c)
-          v[c] = wsum8_any<float>(
-              [&](int rr, int d) { return st[((rr * kNodeWin) + q0 + d) * 6 + c]; }, ab, de, gh);
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+          if (c < 3 ? wantE : wantB)
+            v[c] = wsum8_any<float>(
+                [&](int rr, int d) { return st[((rr * kNodeWin) + q0 + d) * 6 + c]; }, ab, de, gh);
       }
       wave_lds_sync();
     } else if (s < n) {
       const int64_t i1 = 8, j1 = 8 * sy, k1 = 8 * sz;
-      for (int c = wantE ? 0 : 3; c < (wantB ? 6 : 3); ++c) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        if (!(c < 3 ? wantE : wantB)) continue;
         const int64_t o = 8 * b + c;
         auto N = [&](int rr, int d) {
           return R[o + ((rr & 1) ? j1 : 0) + ((rr & 2) ? k1 : 0) + (d ? i1 : 0)];
@@ -462,20 +472,20 @@ __global__ void __launch_bounds__(kThreads)
       }
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) st[lane * kRecPad + c] = v[c];
-    st[lane * kRecPad + 6] = fl ? 1.f : 0.f;
-    st[lane * kRecPad + 7] = 0.f;
+    for (int c = 0; c < 6; ++c) ot[lane * kRecPad + c] = double(v[c]);
+    ot[lane * kRecPad + 6] = fl ? 1.0 : 0.0;
+    ot[lane * kRecPad + 7] = 0.0;
     const uint32_t dst = s < n ? sv[s] : 0u;
     wave_lds_sync();
-    // round j: lanes 2k, 2k+1 store the two halves of record 32 j + k
-    const int part = lane & 1;
+    // round j: lanes 4k..4k+3 store the 4 parts of record 16 j + k
+    const int part = lane & 3;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int r = 32 * j + (lane >> 1);
+    for (int j = 0; j < 4; ++j) {
+      const int r = 16 * j + (lane >> 2);
       const uint32_t d = uint32_t(__shfl(int(dst), r));
       if (wb + r < n) {
-        const float* sr = st + r * kRecPad + 4 * part;
-        reinterpret_cast<float4*>(out + 8 * int64_t(d))[part] = make_float4(sr[0], sr[1], sr[2], sr[3]);
+        const double* sr = ot + r * kRecPad + 2 * part;
+        reinterpret_cast<double2*>(out8 + 8 * int64_t(d))[part] = make_double2(sr[0], sr[1]);
       }
     }
     wave_lds_sync();
@@ -483,46 +493,6 @@ __global__ void __launch_bounds__(kThreads)
   if (mine) atomicAdd(&cnt, mine);
   __syncthreads();
   if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
-}
-
-// the FP32 32-byte records back to particle order, E and B as FP64
-__global__ void __launch_bounds__(kThreads)
-    k_unpack32(const float* __restrict__ out, int64_t n, double* __restrict__ E,
-               double* __restrict__ B, int8_t* __restrict__ clamped) {
-  __shared__ float stage[kThreads / 64][64 * 8];
-  const int lane = threadIdx.x & 63;
-  float* w = stage[threadIdx.x >> 6];
-  const float4* in4 = reinterpret_cast<const float4*>(out);
-  for (int64_t i0 = blockIdx.x * int64_t(blockDim.x) + (threadIdx.x & ~63); i0 < n;
-       i0 += int64_t(gridDim.x) * blockDim.x) {
-    const int nrec = n - i0 < 64 ? int(n - i0) : 64;
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int e = lane + 64 * q;  // 16-byte element of the chunk, record e / 2
-      if ((e >> 1) < nrec) {
-        const float4 v = in4[2 * i0 + e];
-        w[4 * e] = v.x;
-        w[4 * e + 1] = v.y;
-        w[4 * e + 2] = v.z;
-        w[4 * e + 3] = v.w;
-      }
-    }
-    wave_lds_sync();
-    if (E)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const int e = lane + 64 * k;
-        if (e / 3 < nrec) E[3 * i0 + e] = double(w[(e / 3) * 8 + e % 3]);
-      }
-    if (B)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const int e = lane + 64 * k;
-        if (e / 3 < nrec) B[3 * i0 + e] = double(w[(e / 3) * 8 + 3 + e % 3]);
-      }
-    if (clamped && lane < nrec) clamped[i0 + lane] = w[lane * 8 + 6] != 0.f ? 1 : 0;
-    wave_lds_sync();
-  }
 }
 
 // original order: the 64-byte records out8[i] = (E0, E1, E2, B0, B1, B2,
@@ -607,19 +577,19 @@ void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const
     }
     const int64_t n = sp.n;
     if (n <= 0) return;
-    DevBuf out32(c->pool, sizeof(float) * 8 * n);
+    DevBuf out8(c->pool, sizeof(double) * 8 * n);
     const int64_t nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
     {
       ProfScope ps(c, ST_GATHER);
       hipLaunchKernelGGL(k_interp_cells32, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
                          rec32.as<const float>(), int(E && ef), int(B && bf), n,
                          sp.srec.as<const float4>(), sp.sk.as<const uint32_t>(),
-                         sp.sv.as<const uint32_t>(), x, y, z, p, out32.as<float>(), cnt);
+                         sp.sv.as<const uint32_t>(), x, y, z, p, out8.as<double>(), cnt);
       OSC_HIP(hipGetLastError());
     }
     ProfScope po(c, ST_GATHER_OUT);
-    hipLaunchKernelGGL(k_unpack32, dim3(unsigned(nb)), dim3(kThreads), 0, c->stream,
-                       out32.as<const float>(), n, (E && ef) ? E : nullptr,
+    hipLaunchKernelGGL(k_unpack, dim3(unsigned(nb)), dim3(kThreads), kUnpackOccLds, c->stream,
+                       out8.as<const double>(), n, (E && ef) ? E : nullptr,
                        (B && bf) ? B : nullptr, clamped);
     OSC_HIP(hipGetLastError());
     return;
