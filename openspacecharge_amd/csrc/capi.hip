@@ -1,5 +1,6 @@
 // extern "C" entry points (include/opensc_hip.h).
 #include <cmath>
+#include <cstdlib>
 #include <exception>
 
 #include "solvers.h"
@@ -107,6 +108,7 @@ OSC_API int osc_ctx_create(int device, osc_ctx** out) {
   c->hash_host = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(c->mesh_host) +
                                                        kHashHost);
   c->stream = c->own;
+  if (const char* td = std::getenv("OSC_TILED_DEPOSIT")) c->tiled_deposit = td[0] != '0';
   // the device address of the no-allocation gather's scratch on this device
   const int rc = noalloc_setup(device, &c->last_error);
   if (rc != OSC_OK) {

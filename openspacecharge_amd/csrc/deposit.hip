@@ -1,14 +1,20 @@
 // Deterministic cloud-in-cell deposition (deposit_particles,
 // open_spacecharge_mod.f90:156-288; harness twin depose_rho_scalar,
-// test_mod.f90:410-474).
+// test_mod.f90:410-474).  Two paths, both bit-identical run to run (no
+// floating-point atomics):
 //
+// Tiled (charge deposits on meshes whose tile node blocks fit in LDS, the
+// default): particles binned by tiles of cells in one scatter, each tile's
+// nodes summed in LDS as fixed-point integers (deposit_tiled; the kernels
+// and the number format are described at k_tile_count).
+//
+// Cell-sorted (the harness rule, larger meshes):
 //   1. cell key per particle: the upstream FP64 index formula, no contraction
 //   2. stable LSD radix sort of (cell, particle) pairs (wave-ballot ranking)
 //   3. wavefront-segmented reduction of the 8 corner terms over runs of
 //      equal cell keys in the sorted array (fixed shuffle and tree orders)
 //   4. each node sums the <= 8 cell/corner partials that touch it, in a fixed
 //      order.
-// No floating-point atomics: the result is bit-identical run to run.
 #include "osc_internal.h"
 
 namespace osc {
@@ -919,6 +925,467 @@ __global__ void k_node(const A* __restrict__ S, const uint8_t* __restrict__ occ,
   }
 }
 
+// ---------------------------------------------------------------------------
+// tiled deposition (deposit_tiled): the particles binned by tiles of cells,
+// each tile's nodes accumulated in LDS as fixed-point integers
+// ---------------------------------------------------------------------------
+// A tile is 2^tb[0] x 2^tb[1] x 2^tb[2] cells and owns the nodes its cells
+// touch, (2^tb[a] + 1) per axis (neighbouring tiles share a face of nodes).
+//   1. k_tile_count: the bin of every particle (its tile under the deposit's
+//      cell rule; off-mesh particles go to bin ntiles), counted per block
+//   2. k_digit_scan: the exclusive scan of each bin's counts over the blocks;
+//      k_tile_plan: the bins' first slots, the chunks of every tile (work
+//      items of at most `chunk` particles) and the fixed-point scale
+//   3. k_tile_scatter: every particle's record and index to a slot of its
+//      bin (wave-aggregated LDS counters rank the particles of a bin, staged
+//      in LDS so that the stores of a bin's run are contiguous)
+//   4. k_tile_accum: one workgroup per chunk adds the 8 corner terms of each
+//      particle to the tile's nodes in LDS as fixed-point integers and stores
+//      the chunk's node block
+//   5. k_tile_nodes: each node adds the blocks of every chunk of the <= 8
+//      tiles holding it and converts the sum to FP64.
+// Fixed point: every term (the upstream product of weights and charge) is
+// scaled by 2^E, the power of two with n max|q| 2^E < 2^50, and split into
+// its nearest integer and the remaining fraction in units of 2^-32; both are
+// summed in 64-bit integers.  Integer sums do not depend on the order of the
+// additions, so the order in which particles reach a tile (the scatter does
+// not fix it) cannot change rho, which is bit-identical run to run: it is the
+// sum of the terms, each rounded to 2^-82 n max|q|, converted once.  FP32
+// deposition keeps the integer part only (terms rounded to 2^-50 n max|q|).
+constexpr int TS_T = 512, TS_P = 8192;  // threads and particles per count / scatter block
+constexpr int TS_R = 1024;              // particles per scatter round (staged in LDS)
+constexpr int TA_T = 512;               // threads per accumulate workgroup
+constexpr int kTileAccWords = 5632;     // node accumulators of one workgroup (44 KB)
+static_assert(TS_P % TS_R == 0 && TS_R % TS_T == 0, "whole rounds of whole thread steps");
+static_assert(kTileMaxBins % TS_T == 0 && kTileMaxBins <= 2 * 1024, "bins per thread");
+
+__device__ __forceinline__ void tile_mesh(TileParams& p) {
+  if (p.md)
+    for (int d = 0; d < 3; ++d) {
+      p.mn[d] = p.md->mn[d];
+      p.d[d] = p.md->d[d];
+      p.dinv[d] = p.md->dinv[d];
+    }
+}
+
+// the deposit's cell of a particle (all 8 nodes on the mesh) and its bin
+__device__ __forceinline__ uint32_t tile_bin(const TileParams& p, double x, double y, double z,
+                                             int& i0, int& j0, int& k0) {
+  i0 = j0 = k0 = 0;
+  const bool ok = cell_of(x, p.mn[0], p.dinv[0], 1.0, p.nlo[0], p.nn[0], i0) &&
+                  cell_of(y, p.mn[1], p.dinv[1], 1.0, p.nlo[1], p.nn[1], j0) &&
+                  cell_of(z, p.mn[2], p.dinv[2], 1.0, p.nlo[2], p.nn[2], k0);
+  if (!ok) return uint32_t(p.ntiles);
+  return uint32_t(i0 >> p.tb[0]) +
+         uint32_t(p.nt[0]) * (uint32_t(j0 >> p.tb[1]) + uint32_t(p.nt[1]) * uint32_t(k0 >> p.tb[2]));
+}
+
+// Adds the wave's active lanes to cnt[d] with one LDS atomic per distinct d
+// (lanes grouped by a ballot per bit of d) and returns each lane's slot: the
+// counter's value before the addition plus the lane's rank among the wave's
+// lanes of its d.  Every lane of the wave calls it.
+__device__ __forceinline__ uint32_t wave_count(uint32_t* cnt, uint32_t d, bool active, int bits) {
+  const int lane = threadIdx.x & 63;
+  uint64_t peers = __ballot(active);
+  for (int b = 0; b < bits; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bal = __ballot(bit);
+    peers &= bit ? bal : ~bal;
+  }
+  const int leader = peers ? __ffsll((unsigned long long)peers) - 1 : 0;
+  uint32_t base = 0;
+  if (active && lane == leader) base = atomicAdd(&cnt[d], uint32_t(__popcll(peers)));
+  base = uint32_t(__shfl(int(base), leader));
+  return base + uint32_t(__popcll(peers & ((1ull << lane) - 1ull)));
+}
+
+// exclusive scan over a block of TS_T threads of one value per thread (s: TS_T / 64 words)
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t a = __shfl_up(x, o, 64);
+    if (lane >= o) x += a;
+  }
+  if (lane == 63) s[wave] = x;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (int w = 0; w < wave; ++w) pre += s[w];
+  __syncthreads();
+  return pre + x - v;
+}
+
+// step 1: per-block bin counts hist[d * nb + b] (the layout of k_digit_scan),
+// the off-mesh count added to ndrop, max |q| into tp->qmax, and with hpart the
+// block's partial content hash of (x, y, z)
+__global__ void __launch_bounds__(TS_T)
+    k_tile_count(const double* __restrict__ x, const double* __restrict__ y,
+                 const double* __restrict__ z, const double* __restrict__ q, int64_t n,
+                 TileParams p, uint32_t* __restrict__ hist, int nb, uint32_t* __restrict__ ndrop,
+                 TilePlan* __restrict__ tp, uint64_t* __restrict__ hpart) {
+  __shared__ uint32_t h[kTileMaxBins];
+  __shared__ double wmax[TS_T / 64];
+  tile_mesh(p);
+  for (int d = threadIdx.x; d <= p.ntiles; d += TS_T) h[d] = 0u;
+  __syncthreads();
+  double qm = 0.0;
+  uint64_t hs = 0;
+  const int64_t base = int64_t(blockIdx.x) * TS_P;
+  for (int it = 0; it < TS_P / TS_T; ++it) {
+    const int64_t i0w = base + int64_t(it) * TS_T;
+    if (i0w >= n) break;  // uniform per block
+    const int64_t i = i0w + threadIdx.x;
+    const bool act = i < n;
+    uint32_t d = 0;
+    if (act) {
+      const double xi = x[i], yi = y[i], zi = z[i];
+      int a, b, c;
+      d = tile_bin(p, xi, yi, zi, a, b, c);
+      if (q) {
+        const double aq = fabs(q[i]);
+        qm = aq <= 1.7976931348623157e308 ? fmax(qm, aq) : __builtin_inf();
+      }
+      if (hpart) hs += particle_mix(i, xi, yi, zi);
+    }
+    (void)wave_count(h, d, act, p.dbits);
+  }
+  if (hpart) hash_block(hpart, hs);
+  for (int off = 32; off > 0; off >>= 1) qm = fmax(qm, __shfl_xor(qm, off, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = qm;
+  __syncthreads();
+  for (int d = threadIdx.x; d <= p.ntiles; d += TS_T) hist[int64_t(d) * nb + blockIdx.x] = h[d];
+  if (threadIdx.x == 0) {
+    if (h[p.ntiles]) atomicAdd(ndrop, h[p.ntiles]);
+    if (q) {
+      double m = 0.0;
+      for (int w = 0; w < TS_T / 64; ++w) m = fmax(m, wmax[w]);
+      atomicMax(&tp->qmax, (unsigned long long)__double_as_longlong(m));
+    }
+  }
+}
+
+// step 2b: one workgroup of 1024 threads, thread t owning bins 2t and 2t+1
+__global__ void __launch_bounds__(1024)
+    k_tile_plan(const uint32_t* __restrict__ total, int ntiles, int64_t n, double q_each,
+                int hasq, int chunk, TilePlan* __restrict__ tp) {
+  __shared__ uint32_t s[3][16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // per bin: particles, deposit chunks, gather chunks; exclusive scans of the
+  // three over the bins (thread t owns bins 2t and 2t+1)
+  uint32_t c[2][3];
+  for (int k = 0; k < 2; ++k) {
+    const int d = 2 * int(threadIdx.x) + k;
+    const uint32_t cnt = d <= ntiles ? total[d] : 0u;
+    c[k][0] = cnt;
+    c[k][1] = d < ntiles ? (cnt + uint32_t(chunk) - 1u) / uint32_t(chunk) : 0u;
+    c[k][2] = d < ntiles ? (cnt + uint32_t(kTileGatherChunk) - 1u) / uint32_t(kTileGatherChunk) : 0u;
+  }
+  uint32_t pre[3];
+  for (int q = 0; q < 3; ++q) {
+    const uint32_t own = c[0][q] + c[1][q];
+    uint32_t v = own;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t a = __shfl_up(v, o, 64);
+      if (lane >= o) v += a;
+    }
+    if (lane == 63) s[q][wave] = v;
+    pre[q] = v - own;
+  }
+  __syncthreads();
+  for (int q = 0; q < 3; ++q)
+    for (int w = 0; w < wave; ++w) pre[q] += s[q][w];
+  for (int k = 0; k < 2; ++k) {
+    const int d = 2 * int(threadIdx.x) + k;
+    if (d <= ntiles) {
+      tp->bstart[d] = pre[0];
+      tp->wl[d] = pre[1];
+      tp->gwl[d] = pre[2];
+    }
+    for (int q = 0; q < 3; ++q) pre[q] += c[k][q];
+  }
+  if (threadIdx.x == 0) {
+    const double qmax = hasq ? __longlong_as_double((long long)tp->qmax) : fabs(q_each);
+    const double bound = double(n) * qmax;
+    int e = 0;
+    tp->bad = !(bound <= 1.7976931348623157e308);
+    if (tp->bad || bound == 0.0) {
+      e = 50;
+    } else {
+      (void)frexp(bound, &e);  // bound < 2^e
+    }
+    int E = 50 - e;
+    E = E > 1000 ? 1000 : (E < -1000 ? -1000 : E);
+    tp->scale = ldexp(1.0, E);
+    tp->unscale = ldexp(1.0, -E);
+  }
+}
+
+// step 3: records and particle indices in bin order.  FP64: (x, y, z, q);
+// F32: the 16-byte (ab, de, gh, q) of the FP32 deposit and the linear cell
+// key (the sentinel nn0 nn1 nn2 off the mesh) in sk.  The block's TS_P
+// particles go in rounds of TS_R: ranked within the round per bin, placed
+// in LDS in bin order and stored so that consecutive threads write
+// consecutive slots of a bin.  gofs[d] is the next global slot of bin d for
+// this block (the block's share of the bin starts at bstart[d] + col[d nb + b]).
+template <bool F32>
+__global__ void __launch_bounds__(TS_T)
+    k_tile_scatter(const double* __restrict__ x, const double* __restrict__ y,
+                   const double* __restrict__ z, const double* __restrict__ q, int64_t n,
+                   TileParams p, const uint32_t* __restrict__ col, int nb,
+                   const TilePlan* __restrict__ tp, double4* __restrict__ srec,
+                   uint32_t* __restrict__ sv, uint32_t* __restrict__ sk) {
+  constexpr int PER = TS_R / TS_T, BPT = kTileMaxBins / TS_T;
+  __shared__ uint32_t gofs[kTileMaxBins], rc[kTileMaxBins], rs[kTileMaxBins];
+  __shared__ double stage[TS_R * (F32 ? 2 : 4)];
+  __shared__ uint32_t sidx[TS_R], skey[F32 ? TS_R : 1];
+  __shared__ uint16_t sbin[TS_R];
+  __shared__ uint32_t sc[TS_T / 64];
+  tile_mesh(p);
+  for (int d = threadIdx.x; d < kTileMaxBins; d += TS_T) {
+    gofs[d] = d <= p.ntiles ? tp->bstart[d] + col[int64_t(d) * nb + blockIdx.x] : 0u;
+    rc[d] = 0u;
+  }
+  __syncthreads();
+  const uint32_t sentinel = uint32_t(p.nn[0]) * uint32_t(p.nn[1]) * uint32_t(p.nn[2]);
+  const int64_t base = int64_t(blockIdx.x) * TS_P;
+  for (int r = 0; r < TS_P / TS_R; ++r) {
+    const int64_t rbase = base + int64_t(r) * TS_R;
+    if (rbase >= n) break;  // uniform per block
+    const int nround = n - rbase < TS_R ? int(n - rbase) : TS_R;
+    uint32_t dk[PER], rk[PER];
+    double4 rec[PER];
+    uint32_t key[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int64_t i = rbase + k * TS_T + threadIdx.x;
+      const bool act = i < n;
+      dk[k] = 0;
+      rec[k] = make_double4(0.0, 0.0, 0.0, 0.0);
+      key[k] = sentinel;
+      if (act) {
+        const double xi = x[i], yi = y[i], zi = z[i], qi = q ? q[i] : 0.0;
+        int a, b, c;
+        dk[k] = tile_bin(p, xi, yi, zi, a, b, c);
+        if (F32) {
+          float4 w = make_float4(0.f, 0.f, 0.f, float(qi));
+          if (dk[k] < uint32_t(p.ntiles)) {
+            w.x = float(((p.mn[0] - xi) + double(a + p.nlo[0]) * p.d[0]) * p.dinv[0]);
+            w.y = float(((p.mn[1] - yi) + double(b + p.nlo[1]) * p.d[1]) * p.dinv[1]);
+            w.z = float(((p.mn[2] - zi) + double(c + p.nlo[2]) * p.d[2]) * p.dinv[2]);
+            key[k] = uint32_t(a) + uint32_t(p.nn[0]) * (uint32_t(b) + uint32_t(p.nn[1]) * uint32_t(c));
+          }
+          rec[k].x = __hiloint2double(__float_as_int(w.y), __float_as_int(w.x));
+          rec[k].y = __hiloint2double(__float_as_int(w.w), __float_as_int(w.z));
+        } else {
+          rec[k] = make_double4(xi, yi, zi, qi);
+        }
+      }
+      rk[k] = wave_count(rc, dk[k], act, p.dbits);
+    }
+    __syncthreads();
+    // round starts of the bins: exclusive scan of the round counts
+    {
+      uint32_t cnt[BPT], sum = 0;
+#pragma unroll
+      for (int k = 0; k < BPT; ++k) {
+        cnt[k] = rc[BPT * threadIdx.x + k];
+        sum += cnt[k];
+      }
+      uint32_t pre = block_excl_scan(sum, sc);
+#pragma unroll
+      for (int k = 0; k < BPT; ++k) {
+        rs[BPT * threadIdx.x + k] = pre;
+        pre += cnt[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int64_t i = rbase + k * TS_T + threadIdx.x;
+      if (i < n) {
+        const uint32_t ls = rs[dk[k]] + rk[k];
+        if (F32) {
+          stage[2 * ls] = rec[k].x;
+          stage[2 * ls + 1] = rec[k].y;
+          skey[ls] = key[k];
+        } else {
+          stage[4 * ls] = rec[k].x;
+          stage[4 * ls + 1] = rec[k].y;
+          stage[4 * ls + 2] = rec[k].z;
+          stage[4 * ls + 3] = rec[k].w;
+        }
+        sidx[ls] = uint32_t(i);
+        sbin[ls] = uint16_t(dk[k]);
+      }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nround; j += TS_T) {
+      const uint32_t d = sbin[j];
+      const uint32_t g = gofs[d] + (uint32_t(j) - rs[d]);
+      if (F32) {
+        reinterpret_cast<double2*>(srec)[g] = make_double2(stage[2 * j], stage[2 * j + 1]);
+        sk[g] = skey[j];
+      } else {
+        srec[g] = make_double4(stage[4 * j], stage[4 * j + 1], stage[4 * j + 2], stage[4 * j + 3]);
+      }
+      sv[g] = sidx[j];
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kTileMaxBins; d += TS_T) {
+      gofs[d] += rc[d];
+      rc[d] = 0u;
+    }
+    __syncthreads();
+  }
+}
+
+// a term scaled by 2^E split into its nearest integer and the remaining
+// fraction in units of 2^-32 (|t scale| < 2^51; adding 1.5 * 2^52 rounds to an
+// integer, which is then the difference of the bit patterns)
+__device__ __forceinline__ void fx_split(double t, double scale, unsigned long long& hi,
+                                         unsigned long long& lo) {
+  const double M = 6755399441055744.0;
+  const long long mb = __double_as_longlong(M);
+  const double m = t * scale;
+  const double a = m + M;
+  hi = (unsigned long long)(__double_as_longlong(a) - mb);
+  const double f = m - (a - M);
+  lo = (unsigned long long)(__double_as_longlong(f * 4294967296.0 + M) - mb);
+}
+__device__ __forceinline__ unsigned long long fx_round(double t, double scale) {
+  const double M = 6755399441055744.0;
+  return (unsigned long long)(__double_as_longlong(t * scale + M) - __double_as_longlong(M));
+}
+
+// work item b: chunk b - wl[t] of tile t (wl[t] <= b < wl[t+1]); -1 past the last
+__device__ __forceinline__ int tile_of_item(const TilePlan* __restrict__ tp, int ntiles, int b) {
+  if (b >= int(tp->wl[ntiles])) return -1;
+  int lo = 0, hi = ntiles - 1;  // the last tile t with wl[t] <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (int(tp->wl[mid]) <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// step 4: one workgroup per work item
+template <bool F32>
+__global__ void __launch_bounds__(TA_T)
+    k_tile_accum(const double4* __restrict__ srec, const uint32_t* __restrict__ sk,
+                 const uint32_t* __restrict__ total, const TilePlan* __restrict__ tp,
+                 TileParams p, CellParams cp, int chunk, unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long acc[kTileAccWords];
+  __shared__ int s_t;
+  const int b = blockIdx.x;
+  if (threadIdx.x == 0) s_t = tile_of_item(tp, p.ntiles, b);
+  tile_mesh(p);
+  if (cp.md)
+    for (int d = 0; d < 3; ++d) {
+      cp.mn[d] = cp.md->mn[d];
+      cp.d[d] = cp.md->d[d];
+      cp.dinv[d] = cp.md->dinv[d];
+    }
+  const int W = F32 ? 1 : 2;
+  const int nwords = p.nodes * W;
+  for (int j = threadIdx.x; j < nwords; j += TA_T) acc[j] = 0ull;
+  __syncthreads();
+  const int t = s_t;
+  if (t < 0) return;  // uniform
+  const int tx = t % p.nt[0], ty = (t / p.nt[0]) % p.nt[1], tz = t / (p.nt[0] * p.nt[1]);
+  const int ox = tx << p.tb[0], oy = ty << p.tb[1], oz = tz << p.tb[2];
+  const int64_t s0 = int64_t(tp->bstart[t]) + int64_t(b - int(tp->wl[t])) * chunk;
+  const int64_t send = int64_t(tp->bstart[t]) + total[t];
+  const int64_t s1 = s0 + chunk < send ? s0 + chunk : send;
+  const double scale = tp->scale;
+  const int sy = p.tn[0], sz = p.tn[0] * p.tn[1];
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += TA_T) {
+    int i0, j0, k0;
+    if (F32) {
+      const float4 r = reinterpret_cast<const float4*>(srec)[s];
+      const uint32_t key = sk[s];
+      const uint32_t nx = uint32_t(p.nn[0]), ny = uint32_t(p.nn[1]);
+      i0 = int(key % nx);
+      j0 = int((key / nx) % ny);
+      k0 = int(key / (nx * ny));
+      const float ab = r.x, de = r.y, gh = r.z, qq = cp.hasq ? r.w : float(cp.q_each);
+      float tt[8];
+      tt[0] = ab * de * gh * qq;
+      tt[2] = ab * (1.f - de) * gh * qq;
+      tt[6] = ab * (1.f - de) * (1.f - gh) * qq;
+      tt[4] = ab * de * (1.f - gh) * qq;
+      tt[5] = (1.f - ab) * de * (1.f - gh) * qq;
+      tt[7] = (1.f - ab) * (1.f - de) * (1.f - gh) * qq;
+      tt[3] = (1.f - ab) * (1.f - de) * gh * qq;
+      tt[1] = (1.f - ab) * de * gh * qq;
+      const int nb0 = (i0 - ox) + sy * (j0 - oy) + sz * (k0 - oz);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int node = nb0 + (c & 1) + ((c >> 1) & 1) * sy + (c >> 2) * sz;
+        atomicAdd(&acc[node], fx_round(double(tt[c]), scale));
+      }
+    } else {
+      const double4 r = srec[s];
+      (void)tile_bin(p, r.x, r.y, r.z, i0, j0, k0);
+      double tt[8];
+      cic_terms(r.x, r.y, r.z, cp.hasq ? r.w : cp.q_each, cp, i0, j0, k0, tt);
+      const int nb0 = (i0 - ox) + sy * (j0 - oy) + sz * (k0 - oz);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int node = nb0 + (c & 1) + ((c >> 1) & 1) * sy + (c >> 2) * sz;
+        unsigned long long h, l;
+        fx_split(tt[c], scale, h, l);
+        atomicAdd(&acc[2 * node], h);
+        atomicAdd(&acc[2 * node + 1], l);
+      }
+    }
+  }
+  __syncthreads();
+  unsigned long long* out = part + int64_t(b) * nwords;
+  for (int j = threadIdx.x; j < nwords; j += TA_T) out[j] = acc[j];
+}
+
+// step 5: node (i, j, k) of the mesh lies in the tiles a = i >> tb and, on a
+// tile face (i a multiple of the tile width), a - 1, per axis
+template <bool F32>
+__global__ void k_tile_nodes(const unsigned long long* __restrict__ part,
+                             const TilePlan* __restrict__ tp, TileParams p,
+                             double* __restrict__ rho) {
+  const uint32_t nx = uint32_t(p.nn[0]), ny = uint32_t(p.nn[1]);
+  const uint32_t ntot = nx * ny * uint32_t(p.nn[2]);
+  const int W = F32 ? 1 : 2;
+  const double unscale = tp->unscale;
+  const bool bad = tp->bad != 0;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ntot; g += gridDim.x * blockDim.x) {
+    const int k = int(g / (nx * ny)), r = int(g - uint32_t(k) * nx * ny);
+    const int j = r / int(nx), i = r - j * int(nx);
+    const int ci[3] = {i, j, k};
+    int ta[3][2], nta[3];
+    for (int a = 0; a < 3; ++a) {
+      const int t = ci[a] >> p.tb[a];
+      nta[a] = 0;
+      if (t < p.nt[a]) ta[a][nta[a]++] = t;
+      if ((ci[a] & ((1 << p.tb[a]) - 1)) == 0 && t >= 1) ta[a][nta[a]++] = t - 1;
+    }
+    unsigned long long H = 0ull, L = 0ull;
+    for (int u = 0; u < nta[2]; ++u)
+      for (int v = 0; v < nta[1]; ++v)
+        for (int w = 0; w < nta[0]; ++w) {
+          const int tx = ta[0][w], ty = ta[1][v], tz = ta[2][u];
+          const int t = tx + p.nt[0] * (ty + p.nt[1] * tz);
+          const int local = (i - (tx << p.tb[0])) +
+                            p.tn[0] * ((j - (ty << p.tb[1])) + p.tn[1] * (k - (tz << p.tb[2])));
+          for (uint32_t c = tp->wl[t]; c < tp->wl[t + 1]; ++c) {
+            const unsigned long long* blk = part + (int64_t(c) * p.nodes + local) * W;
+            H += blk[0];
+            if (!F32) L += blk[1];
+          }
+        }
+    double v = double((long long)H);
+    if (!F32) v = v + double((long long)L) * 2.3283064365386963e-10;  // 2^-32
+    rho[g] = bad ? __builtin_nan("") : v * unscale;
+  }
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1080,6 +1547,157 @@ int64_t read_counter(osc_ctx* c, int idx) {
   return int64_t(v);
 }
 
+// Tile shape for a mesh of nn nodes: from 8^3 cells, the axis with the most
+// tiles doubled until the tiles and the off-mesh bin fit kTileMaxBins bins;
+// false when the node block of a tile would not fit the deposit's LDS
+// accumulators or the gather's LDS node values (6 components)
+static bool tile_shape(const int nn[3], bool f32, TileParams& p) {
+  int cells[3];
+  for (int a = 0; a < 3; ++a) {
+    cells[a] = nn[a] - 1;
+    if (cells[a] < 1) return false;
+    p.tb[a] = 3;
+  }
+  for (;;) {
+    int64_t nt = 1;
+    for (int a = 0; a < 3; ++a) {
+      p.nt[a] = (cells[a] + (1 << p.tb[a]) - 1) >> p.tb[a];
+      nt *= p.nt[a];
+    }
+    if (nt + 1 <= kTileMaxBins) {
+      p.ntiles = int(nt);
+      break;
+    }
+    int a = 0;
+    for (int b = 1; b < 3; ++b)
+      if (p.nt[b] > p.nt[a]) a = b;
+    if (++p.tb[a] > 8) return false;
+  }
+  p.nodes = 1;
+  for (int a = 0; a < 3; ++a) {
+    p.tn[a] = (1 << p.tb[a]) + 1;
+    p.nodes *= p.tn[a];
+  }
+  p.words = f32 ? 1 : 2;
+  if (p.nodes * p.words > kTileAccWords) return false;
+  if (p.nodes * 7 * (f32 ? 4 : 8) > (f32 ? kTileGatherBytes32 : kTileGatherBytes64)) return false;
+  p.dbits = 0;
+  while ((1 << p.dbits) <= p.ntiles) ++p.dbits;
+  return true;
+}
+
+// The deposit of deposit_cic through tiles (charge mode, not the harness rule)
+static void deposit_tiled(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles* keep,
+                          TileParams tp) {
+  const int64_t n = a.n;
+  const int64_t ntot = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
+  uint32_t* cnt = static_cast<uint32_t*>(c->counters);
+  const bool f32 = a.f32 != 0;
+  for (int d = 0; d < 3; ++d) {
+    tp.mn[d] = a.mn[d];
+    tp.dinv[d] = 1.0 / a.delta[d];  // dxi = 1.d0/dx (:225-227)
+    tp.d[d] = a.delta[d];
+    tp.nlo[d] = a.nlo[d];
+    tp.nn[d] = a.nn[d];
+  }
+  tp.md = a.md;
+  CellParams cp{};
+  for (int d = 0; d < 3; ++d) {
+    cp.mn[d] = a.mn[d];
+    cp.d[d] = a.delta[d];
+    cp.dinv[d] = tp.dinv[d];
+    cp.wofs[d] = a.nlo[d];
+    cp.nn[d] = a.nn[d];
+  }
+  cp.md = a.md;
+  cp.hasq = a.q != nullptr;
+  cp.q_each = a.q_each;
+  const int nbins = tp.ntiles + 1;
+  const int nb = int((n + TS_P - 1) / TS_P);
+  // chunks: about 512 work items for the whole bunch, 1024 to 32768 particles
+  int chunk = 1024;
+  while (chunk < 32768 && int64_t(chunk) * 512 < n) chunk <<= 1;
+  const int64_t nwork = (n + chunk - 1) / chunk + tp.ntiles;
+  const size_t rb = f32 ? sizeof(float4) : sizeof(double4);
+  DevBuf plan(c->pool, sizeof(TilePlan));
+  DevBuf hist(c->pool, sizeof(uint32_t) * size_t(nbins) * size_t(nb > 0 ? nb : 1));
+  DevBuf total(c->pool, sizeof(uint32_t) * kTileMaxBins);
+  DevBuf srec(c->pool, rb * (n + 1)), sv(c->pool, sizeof(uint32_t) * (n + 1));
+  DevBuf sk(c->pool, f32 ? sizeof(uint32_t) * (n + 1) : 0);
+  DevBuf part(c->pool, sizeof(unsigned long long) * size_t(nwork) * size_t(tp.nodes * tp.words));
+  TilePlan* pl = plan.as<TilePlan>();
+  // the max |q| word (the two last 32-bit words of the plan) starts at 0
+  fill_u32(c, reinterpret_cast<uint32_t*>(&pl->qmax), 2, 0u);
+  {
+    ProfScope ps(c, ST_DEP_KEYS);
+    DevBuf hpart(c->pool, a.hash ? sizeof(uint64_t) * size_t(nb) : 0);
+    hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(TS_T), 0, c->stream, a.x, a.y, a.z, a.q, n,
+                       tp, hist.as<uint32_t>(), nb, cnt, pl, hpart.as<uint64_t>());
+    OSC_HIP(hipGetLastError());
+    if (a.hash) {
+      hipLaunchKernelGGL(k_hash_sum, dim3(1), dim3(kThreads), 0, c->stream,
+                         hpart.as<const uint64_t>(), nb, a.hash);
+      OSC_HIP(hipGetLastError());
+    }
+  }
+  {
+    ProfScope ps(c, ST_DEP_SORT);
+    hipLaunchKernelGGL(k_digit_scan, dim3(nbins), dim3(RS_T), 0, c->stream, hist.as<uint32_t>(), nb,
+                       total.as<uint32_t>());
+    OSC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_tile_plan, dim3(1), dim3(1024), 0, c->stream, total.as<const uint32_t>(),
+                       tp.ntiles, n, a.q_each, int(a.q != nullptr), chunk, pl);
+    OSC_HIP(hipGetLastError());
+    if (f32)
+      hipLaunchKernelGGL(k_tile_scatter<true>, dim3(nb), dim3(TS_T), 0, c->stream, a.x, a.y, a.z,
+                         a.q, n, tp, hist.as<const uint32_t>(), nb, pl, srec.as<double4>(),
+                         sv.as<uint32_t>(), sk.as<uint32_t>());
+    else
+      hipLaunchKernelGGL(k_tile_scatter<false>, dim3(nb), dim3(TS_T), 0, c->stream, a.x, a.y, a.z,
+                         a.q, n, tp, hist.as<const uint32_t>(), nb, pl, srec.as<double4>(),
+                         sv.as<uint32_t>(), static_cast<uint32_t*>(nullptr));
+    OSC_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps(c, ST_DEP_CELLS);
+    if (f32)
+      hipLaunchKernelGGL(k_tile_accum<true>, dim3(unsigned(nwork)), dim3(TA_T), 0, c->stream,
+                         srec.as<const double4>(), sk.as<const uint32_t>(),
+                         total.as<const uint32_t>(), pl, tp, cp, chunk,
+                         part.as<unsigned long long>());
+    else
+      hipLaunchKernelGGL(k_tile_accum<false>, dim3(unsigned(nwork)), dim3(TA_T), 0, c->stream,
+                         srec.as<const double4>(), static_cast<const uint32_t*>(nullptr),
+                         total.as<const uint32_t>(), pl, tp, cp, chunk,
+                         part.as<unsigned long long>());
+    OSC_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps(c, ST_DEP_NODES);
+    if (f32)
+      hipLaunchKernelGGL(k_tile_nodes<true>, dim3(nblk(ntot)), dim3(kThreads), 0, c->stream,
+                         part.as<const unsigned long long>(), pl, tp, rho);
+    else
+      hipLaunchKernelGGL(k_tile_nodes<false>, dim3(nblk(ntot)), dim3(kThreads), 0, c->stream,
+                         part.as<const unsigned long long>(), pl, tp, rho);
+    OSC_HIP(hipGetLastError());
+  }
+  if (keep) {
+    keep->n = n;
+    keep->sv = std::move(sv);
+    keep->srec = std::move(srec);
+    keep->f32 = f32;
+    if (f32) keep->sk = std::move(sk);
+    keep->tiled = true;
+    keep->tile = tp;
+    keep->tile.md = nullptr;
+    keep->plan = std::move(plan);
+    keep->total = std::move(total);
+    keep->chunk = chunk;
+    keep->nwork = nwork;
+  }
+}
+
 void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles* keep) {
   const int nx = a.nn[0], ny = a.nn[1], nz = a.nn[2];
   const int64_t ncell = int64_t(nx) * ny * nz;
@@ -1089,6 +1707,13 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   uint32_t* cnt = static_cast<uint32_t*>(c->counters);
   fill_u32(c, cnt, 4, 0u);
   const int64_t n = a.n;
+  {
+    TileParams tp{};
+    if (c->tiled_deposit && !a.harness && n > 0 && tile_shape(a.nn, a.f32 != 0, tp)) {
+      deposit_tiled(c, a, rho, keep, tp);
+      return;
+    }
+  }
   // FP32 deposition (the charge-mode deposit of an FP32 context): 16-byte
   // weight records, FP32 sums
   const bool f32 = a.f32 && !a.harness;

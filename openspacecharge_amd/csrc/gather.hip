@@ -536,6 +536,238 @@ __global__ void __launch_bounds__(kThreads)
     wave_lds_sync();  // read before the next chunk overwrites the stage
   }
 }
+
+// ---------------------------------------------------------------------------
+// tile gather: the particles of the tiled deposit, in bin order
+// ---------------------------------------------------------------------------
+constexpr int TG_T = 1024;
+
+// One workgroup per gather item (kTileGatherChunk particles of one tile's
+// bin): the E and B values of the tile's nodes are staged in LDS as node
+// records of 7 values (FP64; F32: the values of the FP32 node records; the
+// odd stride in 4-byte words spreads random nodes over the LDS banks), then
+// each particle sums its 8 nodes from there in the upstream
+// order and leaves (E, B, clamped, 0) in the 64-byte record out8[sv[s]] for
+// k_unpack.  FP64: cell and weights from the coordinates by the
+// interpolate_field rule, the values of k_interp_cells; F32: the deposit's cell
+// key and FP32 weights, the values of k_interp_cells32.  Tiles holding fewer
+// than kTileGatherStage particles, and particles whose cell lies outside the
+// tile (a gather mesh other than the deposit's), read the planar fields.
+constexpr int TG_PER = kTileGatherChunk / TG_T;
+constexpr int kTileGatherStage = 512;
+static_assert(kTileGatherChunk % TG_T == 0, "whole particles per thread");
+template <bool F32>
+__global__ void __launch_bounds__(TG_T)
+    k_tile_gather(const double* __restrict__ ef, const double* __restrict__ bf, int wantE,
+                  int wantB, TileParams tg, const TilePlan* __restrict__ tp,
+                  const uint32_t* __restrict__ total, const double4* __restrict__ srec,
+                  const uint32_t* __restrict__ sk, const uint32_t* __restrict__ sv, IP p,
+                  double* __restrict__ out8, uint32_t* __restrict__ nclamp) {
+  using A = typename std::conditional<F32, float, double>::type;
+  constexpr int NS = 7;  // LDS values per node record
+  __shared__ double lds_raw[(F32 ? kTileGatherBytes32 : kTileGatherBytes64) / 8];
+  __shared__ int s_t;
+  __shared__ uint32_t cnt;
+  A* L = reinterpret_cast<A*>(lds_raw);
+  if (threadIdx.x == 0) {
+    int t = -1;
+    const int b = blockIdx.x, ntiles = tg.ntiles;
+    if (b < int(tp->gwl[ntiles])) {
+      int lo = 0, hi = ntiles - 1;  // the last tile t with gwl[t] <= b
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (int(tp->gwl[mid]) <= b) lo = mid;
+        else hi = mid - 1;
+      }
+      t = lo;
+    }
+    s_t = t;
+    cnt = 0;
+  }
+  __syncthreads();
+  const int t = s_t;
+  if (t < 0) return;  // uniform
+  const int tx = t % tg.nt[0], ty = (t / tg.nt[0]) % tg.nt[1], tz = t / (tg.nt[0] * tg.nt[1]);
+  const int ox = tx << tg.tb[0], oy = ty << tg.tb[1], oz = tz << tg.tb[2];
+  const int nodes = tg.nodes, tn0 = tg.tn[0], tn01 = tg.tn[0] * tg.tn[1];
+  const int64_t sy = p.nn[0], sz = int64_t(p.nn[0]) * p.nn[1], nt = sz * p.nn[2];
+  const uint32_t count = total[t];
+  const bool staged = count >= uint32_t(kTileGatherStage);  // uniform
+  if (staged) {
+    for (int node = threadIdx.x; node < nodes; node += TG_T) {
+      const int lk = node / tn01, rr = node - lk * tn01, lj = rr / tn0, li = rr - lj * tn0;
+      const int gi = ox + li, gj = oy + lj, gk = oz + lk;
+      const bool in = gi < p.nn[0] && gj < p.nn[1] && gk < p.nn[2];
+      const int64_t g = in ? gi + sy * gj + sz * gk : 0;
+      A v[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const double* F = c < 3 ? ef : bf;
+        v[c] = A((in && (c < 3 ? wantE : wantB)) ? F[(c % 3) * nt + g] : 0.0);
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) L[NS * node + c] = v[c];
+    }
+    __syncthreads();
+  }
+  const int64_t s0 = int64_t(tp->bstart[t]) +
+                     int64_t(int(blockIdx.x) - int(tp->gwl[t])) * kTileGatherChunk;
+  const int64_t send = int64_t(tp->bstart[t]) + count;
+  const int wx = 1 << tg.tb[0], wy = 1 << tg.tb[1], wz = 1 << tg.tb[2];
+  // the thread's particles: records first, so that their loads overlap
+  double4 rec[TG_PER];
+  uint32_t key[TG_PER], dst[TG_PER];
+#pragma unroll
+  for (int k = 0; k < TG_PER; ++k) {
+    const int64_t s = s0 + k * TG_T + threadIdx.x;
+    rec[k] = make_double4(0.0, 0.0, 0.0, 0.0);
+    key[k] = 0;
+    dst[k] = 0;
+    if (s < send) {
+      if (F32) {
+        const float4 r = reinterpret_cast<const float4*>(srec)[s];
+        rec[k] = make_double4(r.x, r.y, r.z, 0.0);
+        key[k] = sk[s];
+      } else {
+        rec[k] = srec[s];
+      }
+      dst[k] = sv[s];
+    }
+  }
+  uint32_t mine = 0;
+#pragma unroll
+  for (int k = 0; k < TG_PER; ++k) {
+    const int64_t s = s0 + k * TG_T + threadIdx.x;
+    if (s >= send) continue;
+    A ab, de, gh;
+    int i0, j0, k0;
+    bool fl = false;
+    if (F32) {
+      const uint32_t nx = uint32_t(p.nn[0]), ny = uint32_t(p.nn[1]);
+      i0 = int(key[k] % nx);
+      j0 = int((key[k] / nx) % ny);
+      k0 = int(key[k] / (nx * ny));
+      ab = A(rec[k].x);
+      de = A(rec[k].y);
+      gh = A(rec[k].z);
+    } else {
+      const double xr = rec[k].x, yr = rec[k].y, zr = rec[k].z;
+      const int ip = clamp_idx(xr, p.mn[0], p.hinv[0], p.nhi[0], fl);
+      const int jp = clamp_idx(yr, p.mn[1], p.hinv[1], p.nhi[1], fl);
+      const int kp = clamp_idx(zr, p.mn[2], p.hinv[2], p.nhi[2], fl);
+      ab = A(((p.mn[0] - xr) + ip * p.delta[0]) * p.hinv[0]);
+      de = A(((p.mn[1] - yr) + jp * p.delta[1]) * p.hinv[1]);
+      gh = A(((p.mn[2] - zr) + kp * p.delta[2]) * p.hinv[2]);
+      i0 = ip - p.nlo[0];
+      j0 = jp - p.nlo[1];
+      k0 = kp - p.nlo[2];
+      i0 = i0 < 0 ? 0 : (i0 > p.nn[0] - 2 ? p.nn[0] - 2 : i0);
+      j0 = j0 < 0 ? 0 : (j0 > p.nn[1] - 2 ? p.nn[1] - 2 : j0);
+      k0 = k0 < 0 ? 0 : (k0 > p.nn[2] - 2 ? p.nn[2] - 2 : k0);
+    }
+    mine += fl ? 1u : 0u;
+    const int li = i0 - ox, lj = j0 - oy, lk = k0 - oz;
+    A v[6] = {A(0), A(0), A(0), A(0), A(0), A(0)};
+    if (staged && li >= 0 && li < wx && lj >= 0 && lj < wy && lk >= 0 && lk < wz) {
+      const int q0 = li + tn0 * lj + tn01 * lk;
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        if (c < 3 ? wantE : wantB)
+          v[c] = wsum8_any<A>(
+              [&](int r2, int d) {
+                return L[NS * (q0 + d + ((r2 & 1) ? tn0 : 0) + ((r2 & 2) ? tn01 : 0)) + c];
+              },
+              ab, de, gh);
+    } else {
+      const int64_t b = i0 + sy * j0 + sz * k0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        if (c < 3 ? wantE : wantB) {
+          const double* F = (c < 3 ? ef : bf) + (c % 3) * nt;
+          v[c] = wsum8_any<A>(
+              [&](int r2, int d) {
+                return A(F[b + d + ((r2 & 1) ? sy : 0) + ((r2 & 2) ? sz : 0)]);
+              },
+              ab, de, gh);
+        }
+    }
+    double2* o = reinterpret_cast<double2*>(out8 + 8 * int64_t(dst[k]));
+    o[0] = make_double2(double(v[0]), double(v[1]));
+    o[1] = make_double2(double(v[2]), double(v[3]));
+    o[2] = make_double2(double(v[4]), double(v[5]));
+    o[3] = make_double2(fl ? 1.0 : 0.0, 0.0);
+  }
+  if (mine) atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
+}
+
+// The off-mesh bin of the tiled deposit (slots bstart[ntiles] on): the
+// interpolate_field clamp rule from the coordinates (FP64: the record; F32:
+// x, y, z of the particle, the weights rounded as in k_interp_cells32),
+// the planar fields read directly.
+template <bool F32>
+__global__ void __launch_bounds__(kThreads)
+    k_tile_gather_off(const double* __restrict__ ef, const double* __restrict__ bf, int wantE,
+                      int wantB, int ntiles, const TilePlan* __restrict__ tp,
+                      const uint32_t* __restrict__ total, const double4* __restrict__ srec,
+                      const uint32_t* __restrict__ sv, const double* __restrict__ x,
+                      const double* __restrict__ y, const double* __restrict__ z, IP p,
+                      double* __restrict__ out8, uint32_t* __restrict__ nclamp) {
+  using A = typename std::conditional<F32, float, double>::type;
+  __shared__ uint32_t cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  const int64_t sy = p.nn[0], sz = int64_t(p.nn[0]) * p.nn[1], nt = sz * p.nn[2];
+  const int64_t s0 = tp->bstart[ntiles], s1 = s0 + total[ntiles];
+  uint32_t mine = 0;
+  for (int64_t s = s0 + blockIdx.x * int64_t(blockDim.x) + threadIdx.x; s < s1;
+       s += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t dst = sv[s];
+    double xi, yi, zi;
+    if (F32) {
+      xi = x[dst];
+      yi = y[dst];
+      zi = z[dst];
+    } else {
+      const double4 r = srec[s];
+      xi = r.x;
+      yi = r.y;
+      zi = r.z;
+    }
+    bool fl = false;
+    const int ip = clamp_idx(xi, p.mn[0], p.hinv[0], p.nhi[0], fl);
+    const int jp = clamp_idx(yi, p.mn[1], p.hinv[1], p.nhi[1], fl);
+    const int kp = clamp_idx(zi, p.mn[2], p.hinv[2], p.nhi[2], fl);
+    const A ab = A(((p.mn[0] - xi) + ip * p.delta[0]) * p.hinv[0]);
+    const A de = A(((p.mn[1] - yi) + jp * p.delta[1]) * p.hinv[1]);
+    const A gh = A(((p.mn[2] - zi) + kp * p.delta[2]) * p.hinv[2]);
+    int i0 = ip - p.nlo[0], j0 = jp - p.nlo[1], k0 = kp - p.nlo[2];
+    i0 = i0 < 0 ? 0 : (i0 > p.nn[0] - 2 ? p.nn[0] - 2 : i0);
+    j0 = j0 < 0 ? 0 : (j0 > p.nn[1] - 2 ? p.nn[1] - 2 : j0);
+    k0 = k0 < 0 ? 0 : (k0 > p.nn[2] - 2 ? p.nn[2] - 2 : k0);
+    const int64_t b = i0 + sy * j0 + sz * k0;
+    A v[6] = {A(0), A(0), A(0), A(0), A(0), A(0)};
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      if (c < 3 ? wantE : wantB) {
+        const double* F = (c < 3 ? ef : bf) + (c % 3) * nt;
+        v[c] = wsum8_any<A>(
+            [&](int rr, int d) { return A(F[b + d + ((rr & 1) ? sy : 0) + ((rr & 2) ? sz : 0)]); },
+            ab, de, gh);
+      }
+    mine += fl ? 1u : 0u;
+    double2* o = reinterpret_cast<double2*>(out8 + 8 * int64_t(dst));
+    o[0] = make_double2(double(v[0]), double(v[1]));
+    o[1] = make_double2(double(v[2]), double(v[3]));
+    o[2] = make_double2(double(v[4]), double(v[5]));
+    o[3] = make_double2(fl ? 1.0 : 0.0, 0.0);
+  }
+  if (mine) atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
+}
+
 }  // namespace
 
 void pack_nodes(osc_ctx* c, const InterpArgs& a, const double* ef, const double* bf, double* rec) {
@@ -564,6 +796,50 @@ void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const
   const int64_t nt = int64_t(a.nn[0]) * a.nn[1] * a.nn[2];
   uint32_t* cnt = static_cast<uint32_t*>(c->counters) + 2;
   fill_u32(c, cnt, 1, 0u);
+  if (sp.tiled) {
+    // bin order of the tiled deposit: each tile's nodes staged in LDS
+    OSC_REQUIRE(!sp.f32 || (x && y && z), OSC_ERR_ARG, "FP32 gather: coordinates needed");
+    const int64_t n = sp.n;
+    if (n <= 0) return;
+    DevBuf out8(c->pool, sizeof(double) * 8 * n);
+    const TilePlan* tpl = sp.plan.as<const TilePlan>();
+    const uint32_t* tot = sp.total.as<const uint32_t>();
+    const int wantE = int(E && ef), wantB = int(B && bf);
+    // gather items: at most one partial chunk per tile beyond the full ones
+    const int64_t gitems = (n + kTileGatherChunk - 1) / kTileGatherChunk + sp.tile.ntiles;
+    {
+      ProfScope ps(c, ST_GATHER);
+      if (sp.f32) {
+        hipLaunchKernelGGL(k_tile_gather<true>, dim3(unsigned(gitems)), dim3(TG_T), 0, c->stream,
+                           ef, bf, wantE, wantB, sp.tile, tpl, tot,
+                           sp.srec.as<const double4>(), sp.sk.as<const uint32_t>(),
+                           sp.sv.as<const uint32_t>(), p, out8.as<double>(), cnt);
+        OSC_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_tile_gather_off<true>, dim3(64), dim3(kThreads), 0, c->stream, ef, bf,
+                           wantE, wantB, sp.tile.ntiles, tpl, tot, sp.srec.as<const double4>(),
+                           sp.sv.as<const uint32_t>(), x, y, z, p, out8.as<double>(), cnt);
+      } else {
+        hipLaunchKernelGGL(k_tile_gather<false>, dim3(unsigned(gitems)), dim3(TG_T), 0,
+                           c->stream, ef, bf, wantE, wantB, sp.tile, tpl, tot,
+                           sp.srec.as<const double4>(), static_cast<const uint32_t*>(nullptr),
+                           sp.sv.as<const uint32_t>(), p, out8.as<double>(), cnt);
+        OSC_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_tile_gather_off<false>, dim3(64), dim3(kThreads), 0, c->stream, ef,
+                           bf, wantE, wantB, sp.tile.ntiles, tpl, tot, sp.srec.as<const double4>(),
+                           sp.sv.as<const uint32_t>(), static_cast<const double*>(nullptr),
+                           static_cast<const double*>(nullptr), static_cast<const double*>(nullptr),
+                           p, out8.as<double>(), cnt);
+      }
+      OSC_HIP(hipGetLastError());
+    }
+    ProfScope po(c, ST_GATHER_OUT);
+    const int64_t nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
+    hipLaunchKernelGGL(k_unpack, dim3(unsigned(nb)), dim3(kThreads), kUnpackOccLds, c->stream,
+                       out8.as<const double>(), n, (E && ef) ? E : nullptr,
+                       (B && bf) ? B : nullptr, clamped);
+    OSC_HIP(hipGetLastError());
+    return;
+  }
   if (sp.f32) {
     // FP32: 32-byte node and result records, the deposit's cells and weights
     OSC_REQUIRE(x && y && z && !packed, OSC_ERR_ARG, "FP32 gather: coordinates needed");
@@ -650,7 +926,7 @@ int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const dou
       // the node records do not depend on the order: packed while the host
       // waits for the hashes
       DevBuf rec(c->pool, sizeof(double) * 8 * nt);
-      pack_nodes(c, a, ef, bf, rec.as<double>());
+      if (!c->kept.tiled) pack_nodes(c, a, ef, bf, rec.as<double>());
       if (deposit_hash_matches(c)) {
         interpolate_sorted(c, a, ef, bf, c->kept, E, B, clamped, rec.as<const double>());
         return 0;
@@ -660,7 +936,8 @@ int64_t interpolate(osc_ctx* c, const InterpArgs& a, const double* ef, const dou
         ProfScope ps(c, ST_GATHER_SORT);
         sort_for_gather(c, a, n, x, y, z, &sp);
       }
-      interpolate_sorted(c, a, ef, bf, sp, E, B, clamped, rec.as<const double>());
+      interpolate_sorted(c, a, ef, bf, sp, E, B, clamped,
+                         c->kept.tiled ? nullptr : rec.as<const double>());
       return 0;
     }
     SortedParticles sp;

@@ -124,16 +124,6 @@ struct ProfRec {
   hipEvent_t a, b;
 };
 
-// particles in cell-sorted order, as left by a deposit (for a following
-// gather) or by sort_for_gather
-struct SortedParticles {
-  int64_t n = 0;
-  DevBuf sv;    // original particle index of each sorted slot (u32)
-  DevBuf srec;  // 32-byte records (x, y, z, q) in sorted order (double4), or with
-                // f32 16-byte records (ab, de, gh, q) of the FP32 deposit (float4)
-  bool f32 = false;
-  DevBuf sk;    // f32: the sorted cell keys (u32; the sentinel for dropped particles)
-};
 
 // a gather of at least this many points runs in cell order
 constexpr int64_t kGatherSortMin = int64_t(1) << 16;
@@ -150,6 +140,53 @@ struct MeshDev {
   double mn[3], mx[3], d[3], dinv[3];
   double charge;
 };
+// Tiles of the tiled deposit and its gather: a tile is 2^tb[0] x 2^tb[1] x
+// 2^tb[2] cells and owns the (2^tb[a] + 1) nodes per axis its cells touch;
+// the particles are binned by tile (bin ntiles: off the mesh).
+constexpr int kTileMaxBins = 2048;
+// LDS of the tile gather's node records (E and B of every node of a tile, 7
+// values per node: FP64 and FP32)
+constexpr int kTileGatherBytes64 = 155648, kTileGatherBytes32 = 79872;
+// particles per work item of the tile gather
+constexpr int kTileGatherChunk = 4096;
+struct TileParams {
+  double mn[3], dinv[3], d[3];
+  int nlo[3], nn[3];
+  int tb[3], nt[3], tn[3];  // log2 tile cells, tiles and tile nodes per axis
+  int ntiles, dbits, nodes, words;
+  const MeshDev* md;  // if set: mn, d and dinv come from here
+};
+// the bins' first slots, the work items (chunks of at most `chunk` particles
+// of one tile) and the deposit's fixed-point scale, built on the device
+struct TilePlan {
+  uint32_t bstart[kTileMaxBins + 1];  // first slot of every bin
+  uint32_t wl[kTileMaxBins + 1];      // first work item of every tile; wl[ntiles] = all
+  uint32_t gwl[kTileMaxBins + 1];     // the same for the gather's items of kTileGatherChunk
+  double scale, unscale;              // 2^E, 2^-E
+  int bad;                            // a charge or the bound n max|q| is not finite
+  int pad;
+  unsigned long long qmax;            // bits of max |q| (non-negative: ordered as integers)
+};
+
+// particles in cell-sorted order (or, `tiled`, binned by tile), as left by a
+// deposit (for a following gather) or by sort_for_gather
+struct SortedParticles {
+  int64_t n = 0;
+  DevBuf sv;    // original particle index of each sorted slot (u32)
+  DevBuf srec;  // 32-byte records (x, y, z, q) in sorted order (double4), or with
+                // f32 16-byte records (ab, de, gh, q) of the FP32 deposit (float4)
+  bool f32 = false;
+  DevBuf sk;    // f32: the sorted cell keys (u32; the sentinel for dropped particles)
+  // tiled: the slots are in bin order (deposit_tiled), not cell order; the
+  // gather then stages each tile's nodes in LDS (tile geometry, plan and bin
+  // counts of the deposit)
+  bool tiled = false;
+  TileParams tile{};
+  DevBuf plan, total;
+  int chunk = 0;
+  int64_t nwork = 0;
+};
+
 constexpr int kMeshDev = 256, kCounterBytes = 512;
 static_assert(kMeshDev + sizeof(MeshDev) <= kCounterBytes, "counter area too small");
 // the context's pinned host block: the mesh copy at 0, the two hashes at
@@ -215,6 +252,10 @@ struct osc_ctx {
   // captured solves of the fused step (osc_ctx_set_graphs), and the device
   // mesh record the Green primitive reads while one is being captured
   bool graphs = false;
+  // charge deposits through tiles of cells and fixed-point node sums in LDS
+  // (deposit_tiled) where the mesh allows; off (OSC_TILED_DEPOSIT=0): the
+  // cell-sorted deposit everywhere
+  bool tiled_deposit = true;
   std::map<std::string, std::unique_ptr<osc::SolveGraph>> sgraphs;
   uint64_t graph_tick = 0;
   int64_t graph_builds = 0, graph_replays = 0;
