@@ -304,8 +304,10 @@ int osc_server_drain(osc_server* server);
  * running at once, requests started on a worker that served the same geometry
  * before, requests started ahead of an older one that did not fit, Green
  * spectrum cache hits, misses (over all workers), bytes the idle workers'
- * contexts hold, the byte budget */
-#define OSC_SERVER_NSTATS 8
+ * contexts hold, the byte budget, times a worker gave pooled buffers back
+ * (each stalls the device) and the bytes given back, device allocations of
+ * the workers' buffer pools and the host microseconds they took */
+#define OSC_SERVER_NSTATS 12
 int osc_server_stats(osc_server* server, int64_t* out, int n);
 const char* osc_server_last_error(const osc_server* server);
 int osc_server_destroy(osc_server* server);

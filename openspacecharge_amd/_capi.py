@@ -29,7 +29,7 @@ OSC_PREC_FP64 = 0
 OSC_PREC_FP32 = 1
 # most mesh nodes osc_interpolate_field_noalloc's fixed scratch holds
 OSC_NOALLOC_MAX_NODES = 1048576
-OSC_SERVER_NSTATS = 8
+OSC_SERVER_NSTATS = 12
 
 
 class LibraryNotBuilt(RuntimeError):

@@ -589,7 +589,7 @@ class Server:
         self.lib.osc_server_drain(self.h)
 
     STATS = ("done", "max_running", "affine_starts", "bypasses", "green_hits", "green_misses",
-             "held_bytes", "budget_bytes")
+             "held_bytes", "budget_bytes", "trims", "trimmed_bytes", "allocs", "alloc_us")
 
     def stats(self) -> dict:
         """Counters since creation (osc_server_stats)."""

@@ -6,6 +6,8 @@
 #include <condition_variable>
 #include <thread>
 
+#include <chrono>
+
 #include "osc_internal.h"
 
 namespace osc {
@@ -13,12 +15,13 @@ namespace osc {
 // ---------------------------------------------------------------------------
 // DevPool
 // ---------------------------------------------------------------------------
+constexpr size_t kReuse = 4;
 static size_t bucket(size_t bytes) {
-  // round to 1/8 of the next power of two (>= 256 B) so shapes reuse buffers
+  // round up to a power of two (>= 256 B): buffers of a few size classes
+  // serve every shape of a mixed stream
   size_t b = 256;
   while (b < bytes) b <<= 1;
-  size_t step = b >> 3;
-  return step ? (bytes + step - 1) / step * step : b;
+  return b;
 }
 
 void* DevPool::get(size_t bytes) {
@@ -40,10 +43,12 @@ void* DevPool::get(size_t bytes) {
       return p;
     }
   }
-  // best fit among cached buffers of at most twice the request, so that
-  // requests of varying sizes (a stream of bunches) reuse memory
+  // best fit among cached buffers of at most kReuse times the request, so
+  // that requests of varying sizes (a stream of bunches of mixed grids) reuse
+  // memory: a device allocation costs tens of microseconds to several
+  // milliseconds (the runtime's state decides), a larger buffer nothing
   auto it = free_.lower_bound(bb);
-  if (it != free_.end() && it->first <= 2 * bb) {
+  if (it != free_.end() && it->first <= kReuse * bb) {
     void* p = it->second;
     const size_t have = it->first;
     free_.erase(it);
@@ -53,6 +58,7 @@ void* DevPool::get(size_t bytes) {
     return p;
   }
   void* p = nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
   hipError_t e = hipMalloc(&p, bb);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -64,6 +70,10 @@ void* DevPool::get(size_t bytes) {
       throw Error{OSC_ERR_NOMEM, "device allocation of " + std::to_string(bb) + " bytes failed"};
     }
   }
+  ++allocs_;
+  alloc_ns_ += uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(
+                            std::chrono::steady_clock::now() - t0)
+                            .count());
   live_[p] = bb;
   live_bytes_ += bb;
   return p;
@@ -274,6 +284,18 @@ void DevPool::release_free() {
   for (auto& kv : free_) (void)hipFree(kv.second);
   free_.clear();
   cached_ = 0;
+}
+
+size_t DevPool::trim(size_t target) {
+  size_t freed = 0;
+  while (!free_.empty() && cached_ + live_bytes_ > target) {
+    auto it = std::prev(free_.end());
+    (void)hipFree(it->second);
+    cached_ -= it->first;
+    freed += it->first;
+    free_.erase(it);
+  }
+  return freed;
 }
 
 void DevPool::release_all() {

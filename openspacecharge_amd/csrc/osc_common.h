@@ -137,8 +137,14 @@ class DevPool {
   void put(void* p);
   void release_free();
   void release_all();
+  // frees cached buffers, largest first, until at most `target` bytes are
+  // cached or live; returns the bytes freed
+  size_t trim(size_t target);
   size_t bytes_cached() const { return cached_; }
   size_t bytes_live() const { return live_bytes_; }
+  // device allocations made, and the host time they took (ns)
+  uint64_t allocs() const { return allocs_; }
+  uint64_t alloc_ns() const { return alloc_ns_; }
   // while set, buffers given back are moved here instead of the free list
   // (the buffers a captured HIP graph keeps using), and get() takes from
   // here first; adopt() returns one to the free list
@@ -150,6 +156,7 @@ class DevPool {
   std::multimap<size_t, void*> free_;  // by size: best fit
   std::unordered_map<void*, size_t> live_;
   size_t cached_ = 0, live_bytes_ = 0;
+  uint64_t allocs_ = 0, alloc_ns_ = 0;
 };
 
 // RAII handle on a pooled device buffer

@@ -14,9 +14,11 @@
 // caller's limit) covers the estimated bytes of the running requests plus
 // what every worker's context holds between requests (Green caches, the pipe
 // Green functions, captured solves, pooled buffers); each worker's Green cache is capped at
-// budget / (8 nworkers) and its pooled buffers are given back when it holds
-// more than budget / (2 nworkers) after a request.  A request larger than the
-// budget runs alone.
+// budget / (8 nworkers).  A worker holding more than budget / (2 nworkers)
+// after a request gives back its largest free pooled buffers until it holds
+// three quarters of that: freeing stalls the whole device and the buffers
+// are allocated again by later requests, so it gives back only what it must.
+// A request larger than the budget runs alone.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -97,6 +99,7 @@ struct osc_server {
   std::vector<int64_t> held;                // per worker: bytes its context holds when idle
   int64_t held_total = 0;
   int64_t n_done = 0, max_running = 0, n_affine = 0, n_bypass = 0;
+  int64_t n_trims = 0, trimmed_bytes = 0;
   std::unordered_map<int64_t, Done> done;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -237,8 +240,11 @@ void osc_server::run_worker(int w) {
     d.run_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
     d.total_ms = std::chrono::duration<double, std::milli>(t_end - job.t_submit).count();
     // what the context keeps between requests: Green cache, pipe Green
-    // functions and pooled buffers; pooled buffers beyond the share go back
-    if (int64_t(c->pool.bytes_cached() + c->pool.bytes_live()) > share) c->pool.release_free();
+    // functions and pooled buffers; past the share, free pooled buffers go
+    // back, largest first, down to three quarters of it
+    size_t trimmed = 0;
+    if (int64_t(c->pool.bytes_cached() + c->pool.bytes_live()) > share)
+      trimmed = c->pool.trim(size_t(share / 4 * 3));
     int64_t keep = int64_t(c->pool.bytes_cached() + c->pool.bytes_live());
     for (auto& kv : c->sgraphs)  // the temporaries of captured solves
       for (auto& pb : kv.second->kept) keep += int64_t(pb.second);
@@ -247,6 +253,10 @@ void osc_server::run_worker(int w) {
       if (d.status != OSC_OK) last_error = c->last_error;
       reserved -= job.est;
       --running;
+      if (trimmed) {
+        ++n_trims;
+        trimmed_bytes += int64_t(trimmed);
+      }
       set_held(w, keep);
       auto& aff = affinity[w];
       bool have = false;
@@ -350,13 +360,16 @@ OSC_API int osc_server_drain(osc_server* s) {
 OSC_API int osc_server_stats(osc_server* s, int64_t* out, int n) {
   if (!s || !out || n < 1) return OSC_ERR_ARG;
   std::lock_guard<std::mutex> lk(s->mu);
-  int64_t hits = 0, misses = 0;
+  int64_t hits = 0, misses = 0, allocs = 0, alloc_us = 0;
   for (auto* c : s->ctxs) {
     hits += int64_t(c->gcache.hits);
     misses += int64_t(c->gcache.misses);
+    allocs += int64_t(c->pool.allocs());
+    alloc_us += int64_t(c->pool.alloc_ns() / 1000);
   }
-  const int64_t v[OSC_SERVER_NSTATS] = {s->n_done,     s->max_running, s->n_affine, s->n_bypass,
-                                        hits,          misses,         s->held_total, s->budget};
+  const int64_t v[OSC_SERVER_NSTATS] = {s->n_done,  s->max_running, s->n_affine,   s->n_bypass,
+                                        hits,       misses,         s->held_total, s->budget,
+                                        s->n_trims, s->trimmed_bytes, allocs, alloc_us};
   for (int i = 0; i < n && i < OSC_SERVER_NSTATS; ++i) out[i] = v[i];
   return OSC_OK;
 }
