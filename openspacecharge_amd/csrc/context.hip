@@ -18,10 +18,13 @@ namespace osc {
 constexpr size_t kReuse = 4;
 static size_t bucket(size_t bytes) {
   // round up to a power of two (>= 256 B): buffers of a few size classes
-  // serve every shape of a mixed stream
+  // serve every shape of a mixed stream; above 1 GiB to a quarter of one
+  // (at most a quarter more memory for the largest buffers)
   size_t b = 256;
   while (b < bytes) b <<= 1;
-  return b;
+  if (b <= (size_t(1) << 30)) return b;
+  const size_t step = b >> 2;
+  return (bytes + step - 1) / step * step;
 }
 
 void* DevPool::get(size_t bytes) {
