@@ -153,3 +153,28 @@ def test_server_reports_request_errors():
     E2 = torch.empty((t2[0].shape[0], 3), dtype=torch.float64, device=t2[0].device)
     srv.wait(srv.submit(*t2, E2, **kw2))
     srv.close()
+
+
+def test_server_stats_report_pools():
+    """The statistics report the workers' device allocations (a running server
+    reuses its pooled buffers: a second pass over the same requests allocates
+    far less than the first) and the pool trims (a budget well below what the
+    workers hold makes them give buffers back; results stay bit-identical)."""
+    reqs = _mix()
+    srv = osc.Server(nworkers=2)
+    a = _run(srv, reqs, osc.OSC_PREC_FP64, concurrent=True)
+    s1 = srv.stats()
+    b = _run(srv, reqs, osc.OSC_PREC_FP64, concurrent=True)
+    s2 = srv.stats()
+    srv.close()
+    assert s1["allocs"] > 0 and s1["alloc_us"] >= 0
+    assert s2["allocs"] - s1["allocs"] < s1["allocs"], (s1, s2)
+    for (Ea, Ba), (Eb, Bb) in zip(a, b):
+        assert np.array_equal(Ea, Eb) and np.array_equal(Ba, Bb)
+    small = osc.Server(nworkers=2, max_bytes=256 << 20)
+    c = _run(small, reqs, osc.OSC_PREC_FP64, concurrent=True)
+    st = small.stats()
+    small.close()
+    assert st["trims"] > 0 and st["trimmed_bytes"] > 0, st
+    for (Ea, Ba), (Ec, Bc) in zip(a, c):
+        assert np.array_equal(Ea, Ec) and np.array_equal(Ba, Bc)
