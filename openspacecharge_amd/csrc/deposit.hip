@@ -1123,47 +1123,75 @@ __global__ void __launch_bounds__(1024)
 // step 3: records and particle indices in bin order.  FP64: (x, y, z, q);
 // F32: the 16-byte (ab, de, gh, q) of the FP32 deposit and the linear cell
 // key (the sentinel nn0 nn1 nn2 off the mesh) in sk.  The block's TS_P
-// particles go in rounds of TS_R: ranked within the round per bin, placed
-// in LDS in bin order and stored so that consecutive threads write
-// consecutive slots of a bin.  gofs[d] is the next global slot of bin d for
-// this block (the block's share of the bin starts at bstart[d] + col[d nb + b]).
-template <bool F32>
-__global__ void __launch_bounds__(TS_T)
+// particles go in rounds of R: ranked within the round per bin, placed in
+// LDS in bin order and stored so that consecutive threads write consecutive
+// slots of a bin.  Rounds are as large as two workgroups per CU allow (2048
+// particles, 1024 with charges in FP64; with about a thousand bins shorter
+// rounds leave most bins' runs at one or two records, partial lines), the
+// particle's bin packed with its index in the block.  gofs[d] is the next
+// global slot of bin d for this block (the block's share of the bin starts
+// at bstart[d] + col[d nb + b]).
+template <int T>
+__device__ __forceinline__ uint32_t block_excl_scan_t(uint32_t v, uint32_t* s) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t xv = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t a2 = __shfl_up(xv, o, 64);
+    if (lane >= o) xv += a2;
+  }
+  if (lane == 63) s[wave] = xv;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (int w = 0; w < wave; ++w) pre += s[w];
+  __syncthreads();
+  return pre + xv - v;
+}
+constexpr int TS3_T = 512;
+template <bool F32, bool HASQ>
+struct ScatterGeo {
+  static constexpr int R = (!F32 && HASQ) ? 1024 : 2048;  // particles per round
+  static constexpr int W = F32 ? 2 : (HASQ ? 4 : 3);      // staged doubles per record
+};
+static_assert(TS_P <= (1 << 13) && kTileMaxBins <= (1 << 19), "bin and index share 32 bits");
+template <bool F32, bool HASQ>
+__global__ void __launch_bounds__(TS3_T)
     k_tile_scatter(const double* __restrict__ x, const double* __restrict__ y,
                    const double* __restrict__ z, const double* __restrict__ q, int64_t n,
                    TileParams p, const uint32_t* __restrict__ col, int nb,
                    const TilePlan* __restrict__ tp, double4* __restrict__ srec,
                    uint32_t* __restrict__ sv, uint32_t* __restrict__ sk) {
-  constexpr int PER = TS_R / TS_T, BPT = kTileMaxBins / TS_T;
-  __shared__ uint32_t gofs[kTileMaxBins], rc[kTileMaxBins], rs[kTileMaxBins];
-  __shared__ double stage[TS_R * (F32 ? 2 : 4)];
-  __shared__ uint32_t sidx[TS_R], skey[F32 ? TS_R : 1];
-  __shared__ uint16_t sbin[TS_R];
-  __shared__ uint32_t sc[TS_T / 64];
+  constexpr int T = TS3_T, R = ScatterGeo<F32, HASQ>::R, W = ScatterGeo<F32, HASQ>::W;
+  constexpr int PER = R / T, BPT = kTileMaxBins / T;
+  static_assert(TS_P % R == 0 && R % T == 0 && kTileMaxBins % T == 0, "whole rounds and bins");
+  __shared__ uint32_t gofs[kTileMaxBins], rc[kTileMaxBins];
+  __shared__ uint16_t rs[kTileMaxBins];
+  __shared__ double stage[R * W];
+  __shared__ uint32_t sbi[R], skey[F32 ? R : 1];  // bin << 13 | index in the block
+  __shared__ uint32_t sc[T / 64];
   tile_mesh(p);
-  for (int d = threadIdx.x; d < kTileMaxBins; d += TS_T) {
+  for (int d = threadIdx.x; d < kTileMaxBins; d += T) {
     gofs[d] = d <= p.ntiles ? tp->bstart[d] + col[int64_t(d) * nb + blockIdx.x] : 0u;
     rc[d] = 0u;
   }
   __syncthreads();
   const uint32_t sentinel = uint32_t(p.nn[0]) * uint32_t(p.nn[1]) * uint32_t(p.nn[2]);
   const int64_t base = int64_t(blockIdx.x) * TS_P;
-  for (int r = 0; r < TS_P / TS_R; ++r) {
-    const int64_t rbase = base + int64_t(r) * TS_R;
+  for (int r = 0; r < TS_P / R; ++r) {
+    const int64_t rbase = base + int64_t(r) * R;
     if (rbase >= n) break;  // uniform per block
-    const int nround = n - rbase < TS_R ? int(n - rbase) : TS_R;
+    const int nround = n - rbase < R ? int(n - rbase) : R;
     uint32_t dk[PER], rk[PER];
-    double4 rec[PER];
+    double rec[PER][4];
     uint32_t key[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int64_t i = rbase + k * TS_T + threadIdx.x;
+      const int64_t i = rbase + k * T + threadIdx.x;
       const bool act = i < n;
       dk[k] = 0;
-      rec[k] = make_double4(0.0, 0.0, 0.0, 0.0);
+      rec[k][0] = rec[k][1] = rec[k][2] = rec[k][3] = 0.0;
       key[k] = sentinel;
       if (act) {
-        const double xi = x[i], yi = y[i], zi = z[i], qi = q ? q[i] : 0.0;
+        const double xi = x[i], yi = y[i], zi = z[i], qi = HASQ ? q[i] : 0.0;
         int a, b, c;
         dk[k] = tile_bin(p, xi, yi, zi, a, b, c);
         if (F32) {
@@ -1174,10 +1202,13 @@ __global__ void __launch_bounds__(TS_T)
             w.z = float(((p.mn[2] - zi) + double(c + p.nlo[2]) * p.d[2]) * p.dinv[2]);
             key[k] = uint32_t(a) + uint32_t(p.nn[0]) * (uint32_t(b) + uint32_t(p.nn[1]) * uint32_t(c));
           }
-          rec[k].x = __hiloint2double(__float_as_int(w.y), __float_as_int(w.x));
-          rec[k].y = __hiloint2double(__float_as_int(w.w), __float_as_int(w.z));
+          rec[k][0] = __hiloint2double(__float_as_int(w.y), __float_as_int(w.x));
+          rec[k][1] = __hiloint2double(__float_as_int(w.w), __float_as_int(w.z));
         } else {
-          rec[k] = make_double4(xi, yi, zi, qi);
+          rec[k][0] = xi;
+          rec[k][1] = yi;
+          rec[k][2] = zi;
+          rec[k][3] = qi;
         }
       }
       rk[k] = wave_count(rc, dk[k], act, p.dbits);
@@ -1191,47 +1222,40 @@ __global__ void __launch_bounds__(TS_T)
         cnt[k] = rc[BPT * threadIdx.x + k];
         sum += cnt[k];
       }
-      uint32_t pre = block_excl_scan(sum, sc);
+      uint32_t pre = block_excl_scan_t<T>(sum, sc);
 #pragma unroll
       for (int k = 0; k < BPT; ++k) {
-        rs[BPT * threadIdx.x + k] = pre;
+        rs[BPT * threadIdx.x + k] = uint16_t(pre);
         pre += cnt[k];
       }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int64_t i = rbase + k * TS_T + threadIdx.x;
+      const int64_t i = rbase + k * T + threadIdx.x;
       if (i < n) {
         const uint32_t ls = rs[dk[k]] + rk[k];
-        if (F32) {
-          stage[2 * ls] = rec[k].x;
-          stage[2 * ls + 1] = rec[k].y;
-          skey[ls] = key[k];
-        } else {
-          stage[4 * ls] = rec[k].x;
-          stage[4 * ls + 1] = rec[k].y;
-          stage[4 * ls + 2] = rec[k].z;
-          stage[4 * ls + 3] = rec[k].w;
-        }
-        sidx[ls] = uint32_t(i);
-        sbin[ls] = uint16_t(dk[k]);
+#pragma unroll
+        for (int w = 0; w < W; ++w) stage[W * ls + w] = rec[k][w];
+        if (F32) skey[ls] = key[k];
+        sbi[ls] = (dk[k] << 13) | uint32_t(i - base);
       }
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < nround; j += TS_T) {
-      const uint32_t d = sbin[j];
+    for (int j = threadIdx.x; j < nround; j += T) {
+      const uint32_t d = sbi[j] >> 13;
       const uint32_t g = gofs[d] + (uint32_t(j) - rs[d]);
       if (F32) {
         reinterpret_cast<double2*>(srec)[g] = make_double2(stage[2 * j], stage[2 * j + 1]);
         sk[g] = skey[j];
       } else {
-        srec[g] = make_double4(stage[4 * j], stage[4 * j + 1], stage[4 * j + 2], stage[4 * j + 3]);
+        srec[g] = make_double4(stage[W * j], stage[W * j + 1], stage[W * j + 2],
+                               HASQ ? stage[W * j + 3] : 0.0);
       }
-      sv[g] = sidx[j];
+      sv[g] = uint32_t(base) + (sbi[j] & 8191u);
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < kTileMaxBins; d += TS_T) {
+    for (int d = threadIdx.x; d < kTileMaxBins; d += T) {
       gofs[d] += rc[d];
       rc[d] = 0u;
     }
@@ -1648,14 +1672,20 @@ static void deposit_tiled(osc_ctx* c, const DepositArgs& a, double* rho, SortedP
     hipLaunchKernelGGL(k_tile_plan, dim3(1), dim3(1024), 0, c->stream, total.as<const uint32_t>(),
                        tp.ntiles, n, a.q_each, int(a.q != nullptr), chunk, pl);
     OSC_HIP(hipGetLastError());
-    if (f32)
-      hipLaunchKernelGGL(k_tile_scatter<true>, dim3(nb), dim3(TS_T), 0, c->stream, a.x, a.y, a.z,
-                         a.q, n, tp, hist.as<const uint32_t>(), nb, pl, srec.as<double4>(),
-                         sv.as<uint32_t>(), sk.as<uint32_t>());
+    uint32_t* skp = f32 ? sk.as<uint32_t>() : nullptr;
+    auto scatter = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(nb), dim3(TS3_T), 0, c->stream, a.x, a.y, a.z, a.q, n, tp,
+                         hist.as<const uint32_t>(), nb, pl, srec.as<double4>(), sv.as<uint32_t>(),
+                         skp);
+    };
+    if (f32 && a.q)
+      scatter(k_tile_scatter<true, true>);
+    else if (f32)
+      scatter(k_tile_scatter<true, false>);
+    else if (a.q)
+      scatter(k_tile_scatter<false, true>);
     else
-      hipLaunchKernelGGL(k_tile_scatter<false>, dim3(nb), dim3(TS_T), 0, c->stream, a.x, a.y, a.z,
-                         a.q, n, tp, hist.as<const uint32_t>(), nb, pl, srec.as<double4>(),
-                         sv.as<uint32_t>(), static_cast<uint32_t*>(nullptr));
+      scatter(k_tile_scatter<false, false>);
     OSC_HIP(hipGetLastError());
   }
   {
