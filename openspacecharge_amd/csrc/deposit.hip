@@ -1359,8 +1359,10 @@ __global__ void __launch_bounds__(TA_T)
         const int node = nb0 + (c & 1) + ((c >> 1) & 1) * sy + (c >> 2) * sz;
         unsigned long long h, l;
         fx_split(tt[c], scale, h, l);
-        atomicAdd(&acc[2 * node], h);
-        atomicAdd(&acc[2 * node + 1], l);
+        // integer parts, then fractions: two arrays (a node's two words
+        // adjacent would put random nodes on half as many LDS banks)
+        atomicAdd(&acc[node], h);
+        atomicAdd(&acc[p.nodes + node], l);
       }
     }
   }
@@ -1400,9 +1402,9 @@ __global__ void k_tile_nodes(const unsigned long long* __restrict__ part,
           const int local = (i - (tx << p.tb[0])) +
                             p.tn[0] * ((j - (ty << p.tb[1])) + p.tn[1] * (k - (tz << p.tb[2])));
           for (uint32_t c = tp->wl[t]; c < tp->wl[t + 1]; ++c) {
-            const unsigned long long* blk = part + (int64_t(c) * p.nodes + local) * W;
+            const unsigned long long* blk = part + int64_t(c) * p.nodes * W + local;
             H += blk[0];
-            if (!F32) L += blk[1];
+            if (!F32) L += blk[p.nodes];
           }
         }
     double v = double((long long)H);
