@@ -201,13 +201,15 @@ int oc_deposit_particles(int64_t n, const double *x, const double *y, const doub
     double ab = ((xmin - x[p]) + ip * dx) * dxi;
     double de = ((ymin - y[p]) + jp * dy) * dyi;
     double gh = ((zmin - z[p]) + kp * dz) * dzi;
-    /* upstream: no bounds check; here particles off the mesh are skipped */
-    if (ip < 1 || jp < 1 || kp < 1 || ip + 1 > nx || jp + 1 > ny || kp + 1 > nz) {
+    /* rho is allocated (nlo:nhi) (:206-208) and indexed rho(ip, jp, kp)
+       (:274): element ip - nlo of the storage.  upstream: no bounds check;
+       here particles whose 8 nodes are not all on the mesh are skipped */
+    int i0 = ip - nlo[0], j0 = jp - nlo[1], k0 = kp - nlo[2];
+    if (i0 < 0 || j0 < 0 || k0 < 0 || i0 + 1 > nx - 1 || j0 + 1 > ny - 1 || k0 + 1 > nz - 1) {
       ++dropped;
       continue;
     }
     if (q) charge1 = q[p];
-    int i0 = ip - 1, j0 = jp - 1, k0 = kp - 1;
 #define R(i, j, k) rho[IDX3(i, j, k, nx, ny)]
     R(i0, j0, k0) = R(i0, j0, k0) + ab * de * gh * charge1;
     R(i0, j0 + 1, k0) = R(i0, j0 + 1, k0) + ab * (1. - de) * gh * charge1;

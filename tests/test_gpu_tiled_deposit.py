@@ -171,3 +171,24 @@ def test_tiled_order_reused_on_another_mesh(ctxs, oracle):
     E, B = osc.interpolate_field(x, y, z, m, E=True, B=True, ctx=ctx)
     Er, Br, _ = oracle.interpolate_field(m.nlo, m.nhi, m.min, m.delta, m.efield, m.bfield, x, y, z)
     assert np.array_equal(E, Er.T) and np.array_equal(B, Br.T)
+
+
+@pytest.mark.parametrize("prec", [osc.OSC_PREC_FP64, osc.OSC_PREC_FP32])
+def test_tiled_deposit_lower_bound_not_one(ctxs, oracle, prec):
+    """A mesh whose lower index bound is not 1 (nlo = (3, 2, 5)): the tiled
+    deposit's cell rule and weights follow upstream's node offsets; rho vs
+    the oracle (FP32 deposition: within 1e-6 of max rho) and the tiled path
+    against the cell-sorted one."""
+    nlo, nhi = (3, 2, 5), (26, 21, 36)
+    x, y, z = gaussian_bunch(120000, (1e-3, 1e-3, 2e-3), seed=71)
+    res = {}
+    for tiled in (True, False):
+        m = osc.Mesh3D(nlo=nlo, nhi=nhi, gamma=GAMMA)
+        osc.deposit_particles(x, y, z, m, total_charge=1e-9, ctx=ctxs[(prec, tiled)])
+        res[tiled] = m
+    d = oracle.deposit_particles(x, y, z, nlo, nhi, total_charge=1e-9)
+    mt = res[True]
+    assert np.array_equal(mt.delta, d["delta"]) and np.array_equal(mt.min, d["min"])
+    tol = 1e-14 if prec == osc.OSC_PREC_FP64 else 1e-6
+    assert relmax(mt.rho, d["rho"]) < tol
+    assert relmax(mt.rho, res[False].rho) < tol

@@ -162,6 +162,25 @@ def test_deposit_golden(oracle):
     assert np.array_equal(rf["rho"], g["depf_rho"])
 
 
+def test_deposit_lower_bound_shifts_storage(oracle):
+    """rho is allocated (nlo:nhi) and indexed by the 1-based cell of the
+    particle (open_spacecharge_mod.f90:206-208,274), so on one fixed mesh
+    raising nlo only moves the start of the storage: rho with nlo = (3, 2, 5)
+    equals rho with nlo = (1, 1, 1) shifted by (2, 1, 4), away from the first
+    planes (which the particles of cells below nlo would also reach)."""
+    x, y, z = gaussian_bunch(4000, (1e-3, 1e-3, 2e-3), 17)
+    nhi = (30, 25, 42)
+    mn, mx = np.array([-4e-3, -3e-3, -8e-3]), np.array([4e-3, 3e-3, 8e-3])
+    dl = (mx - mn) / (np.array(nhi) - 1)
+    a = oracle.deposit_particles(x, y, z, (1, 1, 1), nhi, total_charge=1e-9, resize=False,
+                                 mn=mn, mx=mx, delta=dl)
+    b = oracle.deposit_particles(x, y, z, (3, 2, 5), nhi, total_charge=1e-9, resize=False,
+                                 mn=mn, mx=mx, delta=dl)
+    assert b["rho"].shape == (28, 24, 38)
+    assert np.array_equal(b["rho"][1:, 1:, 1:], a["rho"][3:, 2:, 5:])
+    assert b["dropped"] > a["dropped"]
+
+
 def test_deposit_no_charge(oracle):
     # neither qa nor total_charge: upstream prints and returns (open_spacecharge_mod.f90:249-252)
     x, y, z = gaussian_bunch(50, (1e-3, 1e-3, 1e-3), 1)
