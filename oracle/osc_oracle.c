@@ -972,7 +972,15 @@ void oc_interpolate_field(const int nlo[3], const int nhi[3], const double mn[3]
     double ab = ((mn[0] - x[p]) + ip * delta[0]) * hxi;
     double de = ((mn[1] - y[p]) + jp * delta[1]) * hyi;
     double gh = ((mn[2] - z[p]) + kp * delta[2]) * hzi;
-    int i0 = ip - 1, j0 = jp - 1, k0 = kp - 1, i1 = ip, j1 = jp, k1 = kp;
+    /* the fields are allocated (nlo:nhi) (open_spacecharge_mod.f90:206-208)
+       and read at (ip, jp, kp) (:369-376): element ip - nlo of the storage,
+       kept inside the array where the clamp to [1, nhi-1] leaves a cell
+       below nlo (upstream reads out of bounds there) */
+    int i0 = ip - nlo[0], j0 = jp - nlo[1], k0 = kp - nlo[2];
+    i0 = i0 < 0 ? 0 : (i0 > nx - 2 ? nx - 2 : i0);
+    j0 = j0 < 0 ? 0 : (j0 > ny - 2 ? ny - 2 : j0);
+    k0 = k0 < 0 ? 0 : (k0 > nz - 2 ? nz - 2 : k0);
+    int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
     for (int f = 0; f < 2; ++f) {
       const double *F = f == 0 ? efield : bfield;
       double *O = f == 0 ? E : B;

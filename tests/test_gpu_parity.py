@@ -348,6 +348,26 @@ def test_interpolate_field_bit_exact(ctx, oracle):
     assert np.array_equal(cl, clr) and clr.sum() > 0
 
 
+def test_interpolate_field_lower_bound_not_one(ctx, oracle):
+    """Fields allocated (nlo:nhi) with nlo = (3, 2, 5): read at element
+    ip - nlo (open_spacecharge_mod.f90:369-376), bit-exact with the oracle,
+    clamped points included."""
+    nlo, nhi = (3, 2, 5), (14, 12, 18)
+    rng = np.random.default_rng(12)
+    m = osc.Mesh3D(nlo=nlo, nhi=nhi, min=np.array([-1e-3, -2e-3, 0.5e-3]),
+                   delta=np.array([2e-4, 4e-4, 1e-4]))
+    m.allocate()
+    m.efield[...] = rng.standard_normal(m.efield.shape)
+    m.bfield[...] = rng.standard_normal(m.bfield.shape)
+    x = rng.uniform(-0.8e-3, 1.8e-3, 5000)
+    y = rng.uniform(-2.2e-3, 2.6e-3, 5000)
+    z = rng.uniform(0.4e-3, 2.3e-3, 5000)
+    (E, B), cl = osc.interpolate_field(x, y, z, m, E=True, B=True, ctx=ctx, return_clamped=True)
+    Er, Br, clr = oracle.interpolate_field(m.nlo, m.nhi, m.min, m.delta, m.efield, m.bfield, x, y, z)
+    assert np.array_equal(E, Er.T) and np.array_equal(B, Br.T)
+    assert np.array_equal(cl, clr) and clr.sum() > 0
+
+
 def test_interpolate_after_deposit_takes_kept_order_only_for_same_points(oracle):
     """A deposit of >= 65536 points keeps its cell order; interpolate_field at
     the same points takes it (content hash), at changed points it sorts anew.

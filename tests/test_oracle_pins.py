@@ -181,6 +181,29 @@ def test_deposit_lower_bound_shifts_storage(oracle):
     assert b["dropped"] > a["dropped"]
 
 
+def test_interpolate_lower_bound_shifts_storage(oracle):
+    """The fields are read at the 1-based cell of the point from arrays
+    allocated (nlo:nhi) (open_spacecharge_mod.f90:206-208,369-376): with
+    nlo = (3, 2, 5) on the same mesh the fields are the nlo = 1 fields without
+    their first planes, and every point of a cell >= nlo interpolates the
+    same values."""
+    rng = np.random.default_rng(23)
+    nhi = (14, 12, 16)
+    mn, dl = np.array([-1e-3, -2e-3, 0.5e-3]), np.array([2e-4, 4e-4, 1e-4])
+    shape = tuple(nhi) + (3,)
+    ea = np.asfortranarray(rng.standard_normal(shape))
+    ba = np.asfortranarray(rng.standard_normal(shape))
+    eb = np.asfortranarray(ea[2:, 1:, 4:, :])
+    bb = np.asfortranarray(ba[2:, 1:, 4:, :])
+    # points in cells 3..12, 2..10, 5..14 (1-based)
+    x = mn[0] + dl[0] * rng.uniform(2.0, 12.0, 3000)
+    y = mn[1] + dl[1] * rng.uniform(1.0, 10.0, 3000)
+    z = mn[2] + dl[2] * rng.uniform(4.0, 14.0, 3000)
+    Ea, Ba, _ = oracle.interpolate_field((1, 1, 1), nhi, mn, dl, ea, ba, x, y, z)
+    Eb, Bb, _ = oracle.interpolate_field((3, 2, 5), nhi, mn, dl, eb, bb, x, y, z)
+    assert np.array_equal(Ea, Eb) and np.array_equal(Ba, Bb)
+
+
 def test_deposit_no_charge(oracle):
     # neither qa nor total_charge: upstream prints and returns (open_spacecharge_mod.f90:249-252)
     x, y, z = gaussian_bunch(50, (1e-3, 1e-3, 1e-3), 1)
