@@ -896,13 +896,18 @@ __global__ void __launch_bounds__(kThreads)
 //
 // The 8 flags and the 8 corner sums of a node are read together and the sums
 // of unflagged cells (memory never written this time) are selected away, so
-// a node costs one round of loads.  The accumulator starts at +0 and a sum of
-// doubles never yields -0 in round-to-nearest, so adding a selected +0 leaves
-// it unchanged: the node values are those of the conditional sum.  Cell
-// counts are below 2^31 (deposit_cic), so the index arithmetic is 32-bit.
+// a node costs one round of loads.  With `sparse` (far fewer particles than
+// cells, most cells empty) the flags are read first and only the sums of
+// flagged cells are loaded: two rounds of loads, but the sums of the empty
+// cells, most of the 8 planes, are never fetched.  The accumulator starts at
+// +0 and a sum of doubles never yields -0 in round-to-nearest, so adding a
+// selected +0 leaves it unchanged: the node values are those of the
+// conditional sum either way.  Cell counts are below 2^31 (deposit_cic), so
+// the index arithmetic is 32-bit.
 template <typename A>
 __global__ void k_node(const A* __restrict__ S, const uint8_t* __restrict__ occ, int nx,
-                       int ny, int nz, int scale_after, double scale, double* __restrict__ rho) {
+                       int ny, int nz, int sparse, int scale_after, double scale,
+                       double* __restrict__ rho) {
   const uint32_t ncell = uint32_t(nx) * uint32_t(ny) * uint32_t(nz);
   const uint32_t sxy = uint32_t(nx) * uint32_t(ny);
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
@@ -916,7 +921,15 @@ __global__ void k_node(const A* __restrict__ S, const uint8_t* __restrict__ occ,
       const bool in = i >= uint32_t(di) && j >= uint32_t(dj) && k >= uint32_t(dk);
       const uint32_t cell = c - uint32_t(di) - uint32_t(dj) * uint32_t(nx) - uint32_t(dk) * sxy;
       f[cc] = in ? occ[cell] : uint8_t(0);
-      v[cc] = in ? S[size_t(cc) * ncell + cell] : A(0);
+      if (!sparse) v[cc] = in ? S[size_t(cc) * ncell + cell] : A(0);
+    }
+    if (sparse) {
+#pragma unroll
+      for (int cc = 0; cc < 8; ++cc) {
+        const int di = cc & 1, dj = (cc >> 1) & 1, dk = cc >> 2;
+        const uint32_t cell = c - uint32_t(di) - uint32_t(dj) * uint32_t(nx) - uint32_t(dk) * sxy;
+        v[cc] = f[cc] ? S[size_t(cc) * ncell + cell] : A(0);
+      }
     }
     A acc = A(0);
 #pragma unroll
@@ -1868,14 +1881,16 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   }
   {
     ProfScope ps(c, ST_DEP_NODES);
+    // at most one cell in 8 holds particles: load the sums of flagged cells only
+    const int sparse = n < ncell / 8 ? 1 : 0;
     if (f32)
       hipLaunchKernelGGL(k_node<float>, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,
-                         S.as<const float>(), occ.as<const uint8_t>(), nx, ny, nz, a.harness,
-                         a.scale_after, rho);
+                         S.as<const float>(), occ.as<const uint8_t>(), nx, ny, nz, sparse,
+                         a.harness, a.scale_after, rho);
     else
       hipLaunchKernelGGL(k_node<double>, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,
-                         S.as<const double>(), occ.as<const uint8_t>(), nx, ny, nz, a.harness,
-                         a.scale_after, rho);
+                         S.as<const double>(), occ.as<const uint8_t>(), nx, ny, nz, sparse,
+                         a.harness, a.scale_after, rho);
     OSC_HIP(hipGetLastError());
   }
   if (keep) {

@@ -199,6 +199,20 @@ struct LdsPad {
   static __host__ __device__ constexpr int size(int n) { return n + (n >> shift) + 1; }
 };
 
+// Workgroups are dealt to the 8 XCDs round-robin in dispatch order
+// (workgroup w runs on XCD w % 8), and each XCD has its own L2.  A strided
+// pass reads rows of BX adjacent columns -- 32 or 64 bytes, less than a
+// 128-byte line, at the long lengths -- so workgroups of neighbouring columns
+// read the same lines; dealt round-robin, each XCD fetches each line from
+// HBM.  xcd_block maps dispatch index w of n to a logical workgroup such that
+// every XCD runs one contiguous range of logical workgroups, in order:
+// neighbours then share one L2.  A bijection of [0, n).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t w, uint32_t n) {
+  constexpr uint32_t kXcd = 8;
+  const uint32_t x = w % kXcd, i = w / kXcd, per = n / kXcd, rem = n % kXcd;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
 // ---------------------------------------------------------------------------
 // compile-time engine
 // ---------------------------------------------------------------------------

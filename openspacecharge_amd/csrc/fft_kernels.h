@@ -168,15 +168,19 @@ __global__ void __launch_bounds__((BX * Geo<T, L>::TPL))
   using F = FftTile<T, L, DIR, BX, false>;
   __shared__ cpx<T> lds[F::LDS];
   F f;
-  const int bg = blockIdx.x * BX + f.b;
+  // (bx, by, bz): the logical workgroup, XCD-grouped (xcd_block)
+  const uint32_t gx = gridDim.x, gy = gridDim.y;
+  const uint32_t lw = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+  const int bx = int(lw % gx), by = int((lw / gx) % gy), bz = int(lw / (gx * gy));
+  const int bg = bx * BX + f.b;
   const bool ok = bg < a.nb;
-  const cpx<T>* ib = in + (ok ? bg * a.s_b + blockIdx.y * a.s_o1 + blockIdx.z * a.s_o2 : 0);
+  const cpx<T>* ib = in + (ok ? bg * a.s_b + by * a.s_o1 + bz * a.s_o2 : 0);
   if constexpr (MIR)
     f.load([&](int, int p) { return ok ? axis_in(ib, a, L, p) : mk<T>(0, 0); });
   else
     f.load([&](int, int p) { return (ok && p < a.n_in) ? ib[p * a.s_p] : mk<T>(0, 0); });
   f.run(lds, tw);
-  cpx<T>* ob = out + (ok ? bg * a.d_b + blockIdx.y * a.d_o1 + blockIdx.z * a.d_o2 : 0);
+  cpx<T>* ob = out + (ok ? bg * a.d_b + by * a.d_o1 + bz * a.d_o2 : 0);
   f.store([&](int, int p, cpx<T> v) {
     const int q = p - a.w0;
     if (ok && q >= 0 && q < a.n_out) ob[q * a.d_p] = v;
@@ -466,11 +470,16 @@ __global__ void __launch_bounds__((Geo<T, L>::NT))
   using F = FftTile<T, L, 1, BX, false>;
   __shared__ cpx<T> lds[F::LDS];
   F f;
-  // one output per block (blockIdx.z): measured faster than one block doing
-  // all outputs of a column or an XCD-grouped mapping, despite reading the
-  // rho_hat column once per output
-  const int o = blockIdx.z;
-  const int kx = blockIdx.x * BX + f.b, ky = blockIdx.y;
+  // one output per workgroup: measured faster than one workgroup doing all
+  // outputs of a column.  The outputs of a column block are consecutive
+  // logical workgroups, and each XCD runs a contiguous range of them
+  // (xcd_block), so a column's rho_hat lines, read once per output and term,
+  // and the lines neighbouring column blocks share are fetched into one L2
+  const uint32_t gx = gridDim.x, gy = gridDim.y, no = gridDim.z;
+  const uint32_t lw = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * no);
+  const int o = int(lw % no);
+  const uint32_t u = lw / no;
+  const int kx = int(u % gx) * BX + f.b, ky = int(u / gx);
   const bool ok = kx < a.nb;
   const OutTerms<T>& ot = a.out[o];
   f.load([&](int, int p) { return ok ? combine(rho, a, ot, kx, ky, p) : mk<T>(0, 0); });
@@ -520,11 +529,12 @@ __device__ __forceinline__ cpx<T> zreal_in(const cpx<T>* gl, int64_t sz, const T
 // is never stored as a full spectrum), multiplied by its rho_hat factor into
 // LDS and transformed back.
 //
-// One-dimensional grid, XCD-aware: workgroups are dealt to the 8 XCDs (each
-// with its own L2) in block-id order, so the outputs of one column unit u
-// get block ids 8*nout*(u/8) + 8*o + u%8 -- one XCD, a few dispatches apart --
-// and the second and later reads of the column's rho_hat line hit that L2.
-// Units are (x block, ky) pairs, nxb x blocks per ky.
+// One-dimensional grid, XCD-aware: output o of column unit u is logical
+// workgroup 8*nout*(u/8) + 8*o + u%8 (xcd_block), so every XCD runs all
+// outputs of a contiguous range of units: the second and later reads of a
+// column's rho_hat line, and the neighbouring units' reads of the same
+// 128-byte lines, hit that XCD's L2.  Units are (x block, ky) pairs, nxb x
+// blocks per ky.
 // lines per workgroup of k_zinv_g: in FP32 half the pass rule's, so that
 // twice the workgroups fit the LDS of a CU (measured faster: 0.17 -> 0.155 ms
 // at 256^3; in FP64 the half-line column reads cost more than that gains)
@@ -544,7 +554,8 @@ __global__ void __launch_bounds__((ZGeo<T, L>::NT))
   using FI = FftTile<T, L, 1, BX, false>;
   static_assert(FF::LDS == FI::LDS, "forward and inverse tiles differ");
   __shared__ cpx<T> lds[FF::LDS];
-  const int chunk = blockIdx.x / (8 * a.nout), r = blockIdx.x % (8 * a.nout);
+  const int lw = int(xcd_block(blockIdx.x, gridDim.x));
+  const int chunk = lw / (8 * a.nout), r = lw % (8 * a.nout);
   const int o = r / 8, u = chunk * 8 + r % 8;
   if (u >= nxb * a.Ly) return;  // whole workgroup: past the last unit
   const int xb = u % nxb, ky = u / nxb;
@@ -614,7 +625,8 @@ __global__ void __launch_bounds__((ZGeo<T, L>::NT))
   constexpr int64_t CB = sizeof(cpx<T>);
   static_assert(FF::LDS == FI::LDS, "forward and inverse tiles differ");
   __shared__ cpx<T> lds[FF::LDS];
-  const int xb = blockIdx.x % nxb, ky = blockIdx.x / nxb;
+  const int lw = int(xcd_block(blockIdx.x, gridDim.x));
+  const int xb = lw % nxb, ky = lw / nxb;
   FF fr;
   const int b = fr.b, t = fr.t;
   const bool ok = xb * BX + b < a.nb;

@@ -495,6 +495,91 @@ __global__ void __launch_bounds__(kThreads)
   if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
 }
 
+// The cell-ordered gather of a sparse step (far fewer particles than nodes):
+// each particle reads its 8 nodes from the planar fields where they lie, so
+// that no node records of the whole mesh are packed for a few particles.
+// Cells and weights as in k_interp_cells (FP64: from the record's
+// coordinates) and k_interp_cells32 (F32: the deposit's key and FP32
+// weights; off the mesh, the clamp rule from the coordinates), the same 8
+// terms in the same order over the same node values (F32: rounded to float
+// as the packed records hold them): the results are those of the packed
+// kernels.  (E, B, clamped, 0) leave as the 64-byte record out8[sv[s]] for
+// k_unpack.
+template <bool F32>
+__global__ void __launch_bounds__(kThreads)
+    k_interp_planar(const double* __restrict__ ef, const double* __restrict__ bf, int wantE,
+                    int wantB, int64_t n, const double4* __restrict__ srec,
+                    const uint32_t* __restrict__ sk, const uint32_t* __restrict__ sv,
+                    const double* __restrict__ x, const double* __restrict__ y,
+                    const double* __restrict__ z, IP p, double* __restrict__ out8,
+                    uint32_t* __restrict__ nclamp) {
+  using A = typename std::conditional<F32, float, double>::type;
+  __shared__ uint32_t cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  uint32_t mine = 0;
+  const int64_t sy = p.nn[0], sz = int64_t(p.nn[0]) * p.nn[1], nt = sz * p.nn[2];
+  const uint32_t ncell = uint32_t(nt);
+  for (int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; s < n;
+       s += int64_t(gridDim.x) * blockDim.x) {
+    bool fl = false;
+    A ab, de, gh;
+    int64_t b;
+    const uint32_t key = F32 ? sk[s] : 0u;
+    if (F32 && key < ncell) {
+      const float4 r = reinterpret_cast<const float4*>(srec)[s];
+      ab = A(r.x);
+      de = A(r.y);
+      gh = A(r.z);
+      b = int64_t(key);
+    } else {
+      double xi, yi, zi;
+      if (F32) {
+        const uint32_t i = sv[s];
+        xi = x[i];
+        yi = y[i];
+        zi = z[i];
+      } else {
+        const double4 r = srec[s];
+        xi = r.x;
+        yi = r.y;
+        zi = r.z;
+      }
+      const int ip = clamp_idx(xi, p.mn[0], p.hinv[0], p.nhi[0], fl);
+      const int jp = clamp_idx(yi, p.mn[1], p.hinv[1], p.nhi[1], fl);
+      const int kp = clamp_idx(zi, p.mn[2], p.hinv[2], p.nhi[2], fl);
+      ab = A(((p.mn[0] - xi) + ip * p.delta[0]) * p.hinv[0]);
+      de = A(((p.mn[1] - yi) + jp * p.delta[1]) * p.hinv[1]);
+      gh = A(((p.mn[2] - zi) + kp * p.delta[2]) * p.hinv[2]);
+      int i0 = ip - p.nlo[0], j0 = jp - p.nlo[1], k0 = kp - p.nlo[2];
+      i0 = i0 < 0 ? 0 : (i0 > p.nn[0] - 2 ? p.nn[0] - 2 : i0);
+      j0 = j0 < 0 ? 0 : (j0 > p.nn[1] - 2 ? p.nn[1] - 2 : j0);
+      k0 = k0 < 0 ? 0 : (k0 > p.nn[2] - 2 ? p.nn[2] - 2 : k0);
+      b = i0 + j0 * sy + k0 * sz;
+    }
+    mine += fl ? 1u : 0u;
+    A v[6] = {A(0), A(0), A(0), A(0), A(0), A(0)};
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      if (c < 3 ? wantE : wantB) {
+        const double* F = (c < 3 ? ef : bf) + (c % 3) * nt;
+        v[c] = wsum8_any<A>(
+            [&](int r2, int d) {
+              return A(F[b + d + ((r2 & 1) ? sy : 0) + ((r2 & 2) ? sz : 0)]);
+            },
+            ab, de, gh);
+      }
+    double2* o = reinterpret_cast<double2*>(out8 + 8 * int64_t(sv[s]));
+    o[0] = make_double2(double(v[0]), double(v[1]));
+    o[1] = make_double2(double(v[2]), double(v[3]));
+    o[2] = make_double2(double(v[4]), double(v[5]));
+    o[3] = make_double2(fl ? 1.0 : 0.0, 0.0);
+  }
+  if (mine) atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
+}
+
 // original order: the 64-byte records out8[i] = (E0, E1, E2, B0, B1, B2,
 // flag, pad) into E(3,n), B(3,n) and the clamp flags.  Each wave moves 64
 // records through LDS: read as 4 contiguous KB, written as contiguous runs of
@@ -837,6 +922,37 @@ void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const
     hipLaunchKernelGGL(k_unpack, dim3(unsigned(nb)), dim3(kThreads), kUnpackOccLds, c->stream,
                        out8.as<const double>(), n, (E && ef) ? E : nullptr,
                        (B && bf) ? B : nullptr, clamped);
+    OSC_HIP(hipGetLastError());
+    return;
+  }
+  if (sp.n < nt / 8 && !packed) {
+    // few particles for the mesh: node values from the planar fields, no
+    // node records packed
+    OSC_REQUIRE(!sp.f32 || (x && y && z), OSC_ERR_ARG, "FP32 gather: coordinates needed");
+    const int64_t n = sp.n;
+    if (n <= 0) return;
+    DevBuf out8(c->pool, sizeof(double) * 8 * n);
+    const int64_t nb = std::min<int64_t>((n + kThreads - 1) / kThreads, 65536 * 8);
+    const int wantE = int(E && ef), wantB = int(B && bf);
+    {
+      ProfScope ps(c, ST_GATHER);
+      if (sp.f32)
+        hipLaunchKernelGGL(k_interp_planar<true>, dim3(unsigned(nb)), dim3(kThreads), 0,
+                           c->stream, ef, bf, wantE, wantB, n, sp.srec.as<const double4>(),
+                           sp.sk.as<const uint32_t>(), sp.sv.as<const uint32_t>(), x, y, z, p,
+                           out8.as<double>(), cnt);
+      else
+        hipLaunchKernelGGL(k_interp_planar<false>, dim3(unsigned(nb)), dim3(kThreads), 0,
+                           c->stream, ef, bf, wantE, wantB, n, sp.srec.as<const double4>(),
+                           static_cast<const uint32_t*>(nullptr), sp.sv.as<const uint32_t>(),
+                           static_cast<const double*>(nullptr), static_cast<const double*>(nullptr),
+                           static_cast<const double*>(nullptr), p, out8.as<double>(), cnt);
+      OSC_HIP(hipGetLastError());
+    }
+    ProfScope po(c, ST_GATHER_OUT);
+    hipLaunchKernelGGL(k_unpack, dim3(unsigned(nb)), dim3(kThreads), kUnpackOccLds, c->stream,
+                       out8.as<const double>(), n, wantE ? E : nullptr, wantB ? B : nullptr,
+                       clamped);
     OSC_HIP(hipGetLastError());
     return;
   }
