@@ -89,8 +89,12 @@ template <typename T>
 __host__ __device__ __forceinline__ cpx<T> sub(cpx<T> a, cpx<T> b) {
   return mk(a.x - b.x, a.y - b.y);
 }
+// Complex products (FFT twiddles, spectral products) may contract into FMAs:
+// one rounding fewer per part, against the -ffp-contract=off that keeps the
+// Green and index arithmetic in upstream's operation order.
 template <typename T>
 __host__ __device__ __forceinline__ cpx<T> mul(cpx<T> a, cpx<T> b) {
+#pragma clang fp contract(fast)
   return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 template <typename T>
