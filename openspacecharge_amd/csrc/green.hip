@@ -35,7 +35,7 @@ inline int64_t nblk(int64_t n, int t = kThreads) {
 __global__ void __launch_bounds__(64)
     k_prim_offset(double* __restrict__ F, int L0, int L1, int L2, double dx, double dy, double dz,
                   double umin, double vmin, double wmin, double factor, int icomp, int mev,
-                  int mod, const MeshDev* __restrict__ md, double gamma, int img) {
+                  int mod, int half, const MeshDev* __restrict__ md, double gamma, int img) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
   if (md) {
     dx = md->d[0];
@@ -65,18 +65,24 @@ __global__ void __launch_bounds__(64)
   const int64_t sy = L0, sz = int64_t(L0) * L1;
   const int64_t at = i + sy * j + sz * k;
   F[at] = g;
-  int64_t ev = -1, od = -1;  // the mirror index along mev / mod
+  // mirror images along an axis of `half` (bit a) are stored only for the
+  // first plane past the lower half, the one plane of them the corner
+  // difference of the lower half reads
+  int64_t ev = -1;  // the mirror index along mev
+  bool wev = false;
   if (mev >= 0) {
     const int c = mev == 0 ? i : (mev == 1 ? j : k);
     ev = at + (L[mev] - 1 - 2 * c) * (mev == 0 ? 1 : (mev == 1 ? sy : sz));
-    F[ev] = g;
+    wev = !((half >> mev) & 1) || c == L[mev] / 2 - 1;
+    if (wev) F[ev] = g;
   }
   if (mod >= 0) {
     const int c = mod == 0 ? i : (mod == 1 ? j : k);
     const int64_t step = (L[mod] - 1 - 2 * c) * (mod == 0 ? 1 : (mod == 1 ? sy : sz));
-    od = at + step;
-    F[od] = -g;
-    if (ev >= 0) F[ev + step] = -g;
+    if (!((half >> mod) & 1) || c == L[mod] / 2 - 1) {
+      F[at + step] = -g;
+      if (wev) F[ev + step] = -g;
+    }
   }
 }
 
@@ -262,8 +268,12 @@ static void prim_mirrors(const GreenSpec& s, bool fast, int& mev, int& mod) {
   }
 }
 
-// primitive F of an offset-kind Green function (open_spacecharge_mod.f90:662-691)
-static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F, bool fast) {
+// primitive F of an offset-kind Green function (open_spacecharge_mod.f90:662-691).
+// lower_halves: the reader (the corner-difference x pass) loads only the
+// lower half, plus one plane, along mirrored y and z axes (unshifted z), so
+// the mirror images beyond that are not stored.
+static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F, bool fast,
+                                   bool lower_halves = false) {
   const double dx = s.delta[0], dy = s.delta[1], dz = s.delta[2] * s.gamma;
   const double factor =
       (s.icomp == 1 || s.icomp == 2) ? s.gamma / (dx * dy * dz) : 1.0 / (dx * dy * dz);
@@ -290,8 +300,13 @@ static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F, bo
   const MeshDev* md = c->green_md;
   OSC_REQUIRE(!md || (s.offset[0] == 0.0 && s.offset[1] == 0.0), OSC_ERR_ARG,
               "device-side Green parameters: unsupported offset");
+  int half = 0;
+  if (lower_halves) {
+    if (mev == 1 || mod == 1) half |= 2;
+    if ((mev == 2 || mod == 2) && s.out_shift_z == 0) half |= 4;
+  }
   launch(k_prim_offset, gp, dim3(64), 0, c->stream, F, s.L[0], s.L[1], s.L[2], dx, dy,
-                     dz, umin, vmin, wmin, factor, s.icomp, mev, mod, md, s.gamma,
+                     dz, umin, vmin, wmin, factor, s.icomp, mev, mod, half, md, s.gamma,
                      int(md && s.out_shift_z != 0));
   OSC_HIP(hipGetLastError());
 }
@@ -436,7 +451,7 @@ GreenRef green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) {
     // its rows, so the difference array is never stored
     {
       ProfScope ps(c, ST_GREEN_EVAL);
-      build_offset_primitive(c, s, g.as<double>(), fast);
+      build_offset_primitive(c, s, g.as<double>(), fast, true);
     }
     // the corner difference of a primitive that is even along an axis is odd
     // about L/2 - 1 there, and even where the primitive is odd
