@@ -247,19 +247,48 @@ __device__ __forceinline__ RowPair row_pair(int64_t ra, int ny_rows) {
 // (ny_rows < L1) and z.
 struct CornerDiff {
   int L1, L2, sh, zero_raw;
+  // far field (far != 0): an interior row whose cells lie at |w| >= wnear
+  // from the origin is not differenced from the primitive but takes each
+  // cell's integral of the field kernel x_a / r^3 (a = icomp - 1) from its
+  // second-order expansion about the cell centre, times factor; h: the cell
+  // edges, c0: the centre coordinates of cell 0 along each axis
+  int far, icomp;
+  double wnear, factor;
+  double h[3], c0[3];
 };
 
-// row (j, kq) of the corner-difference source: its F row and whether it is
-// interior
+// row (j, kq) of the corner-difference source: its F row, whether it is
+// interior and, for an interior row in the far field, far = true
 __device__ __forceinline__ const double* diff_row(const double* F, const R2CIO& a,
                                                   const CornerDiff& cd, int j, int kq, bool ok,
-                                                  bool& inner) {
+                                                  bool& inner, bool& far) {
   inner = false;
+  far = false;
   if (!ok) return F;
   int k = kq - cd.sh;  // kq < L2 and 0 <= sh < L2: one wrap at most
   if (k < 0) k += cd.L2;
   inner = j < cd.L1 - 1 && k < cd.L2 - 1;
+  far = cd.far && inner && fabs(cd.c0[2] + k * cd.h[2]) >= cd.wnear;
   return F + j * a.in_sy + k * a.in_sz;
+}
+
+// the cell integral of x_a / r^3 over the cell centred at (u, v, w) with
+// edges h, to second order: V (f + (hu^2 f_uu + hv^2 f_vv + hw^2 f_ww) / 24),
+// the second derivatives 15 x_a x_b^2 / r^7 - 3 x_a / r^5 (b != a) and
+// 15 x_a^3 / r^7 - 9 x_a / r^5 (b = a); times factor
+__device__ __forceinline__ double far_cell(const CornerDiff& cd, double u, double v, double w) {
+  const double r2 = u * u + v * v + w * w;
+  const double ir2 = 1.0 / r2, ir3 = ir2 / sqrt(r2);
+  const double xa = cd.icomp == 1 ? u : (cd.icomp == 2 ? v : w);
+  const double f = xa * ir3;
+  const double t15 = 15.0 * f * ir2 * ir2, t3 = 3.0 * f * ir2;
+  double fuu = t15 * u * u - t3, fvv = t15 * v * v - t3, fww = t15 * w * w - t3;
+  if (cd.icomp == 1) fuu -= 2.0 * t3;
+  else if (cd.icomp == 2) fvv -= 2.0 * t3;
+  else fww -= 2.0 * t3;
+  const double hu = cd.h[0], hv = cd.h[1], hw = cd.h[2];
+  return cd.factor * (hu * hv * hw) *
+         (f + (hu * hu * fuu + hv * hv * fvv + hw * hw * fww) * (1.0 / 24.0));
 }
 // element p of a corner-difference row, with the terms in the order of the
 // upstream loop (a left-to-right chain of adds and subtracts)
@@ -286,14 +315,35 @@ __global__ void __launch_bounds__((XGeo<T, L>::NT))
   const bool oka = ra < a.nrows, okb = rb < a.nrows;
   const RowPair rp = row_pair(ra, a.ny_rows);
   if constexpr (DIFF) {
-    bool ina, inb;
-    const double* pa = diff_row(in, a, cd, rp.ya, rp.za, oka, ina);
-    const double* pb = diff_row(in, a, cd, rp.yb, rp.zb, okb, inb);
-    f.load([&](int, int p) {
-      T re = (oka && p < a.n_in) ? T(diff_elem(pa, a, cd, ina, p)) : T(0);
-      T im = (okb && p < a.n_in) ? T(diff_elem(pb, a, cd, inb, p)) : T(0);
-      return mk(re, im);
-    });
+    bool ina, inb, fara, farb;
+    const double* pa = diff_row(in, a, cd, rp.ya, rp.za, oka, ina, fara);
+    const double* pb = diff_row(in, a, cd, rp.yb, rp.zb, okb, inb, farb);
+    if (cd.far && (fara || farb)) {
+      // far-field cell integrals (interior cells; the last cell of a row is
+      // raw, i.e. zero, as zero_raw makes it)
+      const double va = cd.c0[1] + rp.ya * cd.h[1], vb = cd.c0[1] + rp.yb * cd.h[1];
+      int ka = rp.za - cd.sh, kb = rp.zb - cd.sh;
+      if (ka < 0) ka += cd.L2;
+      if (kb < 0) kb += cd.L2;
+      const double wa = cd.c0[2] + ka * cd.h[2], wb = cd.c0[2] + kb * cd.h[2];
+      f.load([&](int, int p) {
+        const double u = cd.c0[0] + p * cd.h[0];
+        T re = T(0), im = T(0);
+        if (oka && p < a.n_in)
+          re = fara ? (p < a.n_in - 1 ? T(far_cell(cd, u, va, wa)) : T(0))
+                    : T(diff_elem(pa, a, cd, ina, p));
+        if (okb && p < a.n_in)
+          im = farb ? (p < a.n_in - 1 ? T(far_cell(cd, u, vb, wb)) : T(0))
+                    : T(diff_elem(pb, a, cd, inb, p));
+        return mk(re, im);
+      });
+    } else {
+      f.load([&](int, int p) {
+        T re = (oka && p < a.n_in) ? T(diff_elem(pa, a, cd, ina, p)) : T(0);
+        T im = (okb && p < a.n_in) ? T(diff_elem(pb, a, cd, inb, p)) : T(0);
+        return mk(re, im);
+      });
+    }
   } else {
     const double* pa = in + (oka ? int64_t(rp.ya) * a.in_sy + int64_t(rp.za) * a.in_sz : 0);
     const double* pb = in + (okb ? int64_t(rp.yb) * a.in_sy + int64_t(rp.zb) * a.in_sz : 0);

@@ -35,8 +35,9 @@ inline int64_t nblk(int64_t n, int t = kThreads) {
 __global__ void __launch_bounds__(64)
     k_prim_offset(double* __restrict__ F, int L0, int L1, int L2, double dx, double dy, double dz,
                   double umin, double vmin, double wmin, double factor, int icomp, int mev,
-                  int mod, int half, const MeshDev* __restrict__ md, double gamma, int img) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+                  int mod, int half, int zlo, int zhi, int zbase,
+                  const MeshDev* __restrict__ md, double gamma, int img) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y, k = zbase + int(blockIdx.z);
   if (md) {
     dx = md->d[0];
     dy = md->d[1];
@@ -53,6 +54,11 @@ __global__ void __launch_bounds__(64)
   }
   const bool m0 = mev == 0 || mod == 0, m1 = mev == 1 || mod == 1, m2 = mev == 2 || mod == 2;
   if (i >= (m0 ? L0 / 2 : L0)) return;
+  // only the planes [zlo, zhi] are read (the near cells of a far-field build):
+  // a plane outside is evaluated when its mirror image is inside
+  if (k < zlo || k > zhi) {
+    if (!m2 || L2 - 1 - k < zlo || L2 - 1 - k > zhi) return;
+  }
   const double w = m2 ? (double(k) + (0.5 - double(L2 / 2))) * dz : double(k) * dz + wmin;
   const double v = m1 ? (double(j) + (0.5 - double(L1 / 2))) * dy : double(j) * dy + vmin;
   const double u = m0 ? (double(i) + (0.5 - double(L0 / 2))) * dx : double(i) * dx + umin;
@@ -273,7 +279,8 @@ static void prim_mirrors(const GreenSpec& s, bool fast, int& mev, int& mod) {
 // lower half, plus one plane, along mirrored y and z axes (unshifted z), so
 // the mirror images beyond that are not stored.
 static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F, bool fast,
-                                   bool lower_halves = false) {
+                                   bool lower_halves = false, int zlo = 0, int zhi = -1) {
+  if (zhi < 0 && zlo == 0) zhi = s.L[2] - 1;  // every plane
   const double dx = s.delta[0], dy = s.delta[1], dz = s.delta[2] * s.gamma;
   const double factor =
       (s.icomp == 1 || s.icomp == 2) ? s.gamma / (dx * dy * dz) : 1.0 / (dx * dy * dz);
@@ -293,7 +300,15 @@ static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F, bo
   dim3 gp = g3;
   if (mev == 0 || mod == 0) gp.x = unsigned((s.L[0] / 2 + 63) / 64);
   if (mev == 1 || mod == 1) gp.y = unsigned(s.L[1] / 2);
-  if (mev == 2 || mod == 2) gp.z = unsigned(s.L[2] / 2);
+  // the planes launched: those of [zlo, zhi], or of the lower half whose
+  // mirror images are in it
+  int zbase = zlo, ztop = zhi;
+  if (mev == 2 || mod == 2) {
+    zbase = std::max(0, std::min(zlo, s.L[2] - 1 - zhi));
+    ztop = std::min(s.L[2] / 2 - 1, std::max(zhi, s.L[2] - 1 - zlo));
+  }
+  if (ztop < zbase) return;
+  gp.z = unsigned(ztop - zbase + 1);
   // a solve being captured for replay takes the spacing from the device mesh
   // record; only the zero offset of the free solve and the image offset of
   // the cathode solve occur there
@@ -306,8 +321,8 @@ static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F, bo
     if ((mev == 2 || mod == 2) && s.out_shift_z == 0) half |= 4;
   }
   launch(k_prim_offset, gp, dim3(64), 0, c->stream, F, s.L[0], s.L[1], s.L[2], dx, dy,
-                     dz, umin, vmin, wmin, factor, s.icomp, mev, mod, half, md, s.gamma,
-                     int(md && s.out_shift_z != 0));
+                     dz, umin, vmin, wmin, factor, s.icomp, mev, mod, half, zlo, zhi, zbase,
+                     md, s.gamma, int(md && s.out_shift_z != 0));
   OSC_HIP(hipGetLastError());
 }
 
@@ -412,6 +427,48 @@ void convert_spectrum(osc_ctx* c, const cpx<double>* in, cpx<float>* out, int64_
   OSC_HIP(hipGetLastError());
 }
 
+// Far-field cells of the FP32 build (CornerDiff::far): interior cells whose
+// centres lie at |w| >= kFarEdges times the longest cell edge from the origin
+// take the second-order expansion of the field kernel's cell integral in the
+// x pass, and only the primitive planes the remaining near cells difference
+// are evaluated.  Against 8-point Gauss quadrature of the kernel the
+// expansion's worst error is about 1e-6 of the local field scale V / r^2 at
+// 12 edges for cells 40 times longer in w than across (a relativistic
+// bunch's), less for rounder cells; the FP64 primitive differences carry
+// cancellation errors of 1e-5 .. 1e-4 relative there.  Not in a context
+// that captures solves (osc_ctx_set_graphs: a captured solve knows the
+// spacing only on the device, and its eager and replayed steps must agree).
+// Returns the primitive planes needed, [zlo, zhi] (empty when zlo > zhi).
+constexpr double kFarEdges = 12.0;
+static bool far_field_applies(const osc_ctx* c, const GreenSpec& s, bool fast) {
+  return fast && c->far_green && !c->graphs && s.zero_raw && s.icomp >= 1 && !c->green_md;
+}
+static void far_field_setup(osc_ctx* c, const GreenSpec& s, bool fast, CornerDiff& cd, int& zlo,
+                            int& zhi) {
+  zlo = 0;
+  zhi = s.L[2] - 1;
+  if (!far_field_applies(c, s, fast)) return;
+  const double dx = s.delta[0], dy = s.delta[1], dw = s.delta[2] * s.gamma;
+  cd.far = 1;
+  cd.icomp = s.icomp;
+  cd.factor = (s.icomp == 1 || s.icomp == 2) ? s.gamma / (dx * dy * dw) : 1.0 / (dx * dy * dw);
+  cd.h[0] = dx;
+  cd.h[1] = dy;
+  cd.h[2] = dw;
+  cd.c0[0] = (1.0 - double(s.L[0] / 2)) * dx + s.offset[0];
+  cd.c0[1] = (1.0 - double(s.L[1] / 2)) * dy + s.offset[1];
+  cd.c0[2] = (1.0 - double(s.L[2] / 2)) * dw + s.offset[2] * s.gamma;
+  cd.wnear = kFarEdges * std::max(dx, std::max(dy, dw));
+  int lo = s.L[2], hi = -1;
+  for (int k = 0; k < s.L[2] - 1; ++k)
+    if (!(std::fabs(cd.c0[2] + k * cd.h[2]) >= cd.wnear)) {
+      lo = std::min(lo, k);
+      hi = std::max(hi, k + 1);
+    }
+  zlo = lo;
+  zhi = hi;
+}
+
 // Spectrum of the Green function of s in the context's precision: the real
 // Green function is always built in FP64 (IGF primitives and 8-corner
 // differences); an FP32 context transforms it in FP32.  When the z length has
@@ -423,6 +480,8 @@ GreenRef green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) {
   const SpecDims d = SpecDims::of(s.L);
   const int64_t N = int64_t(s.L[0]) * s.L[1] * s.L[2];
   std::string key(reinterpret_cast<const char*>(&s), sizeof(s));
+  // spectra built with far-field cells are kept apart from exact ones
+  if (s.kind == GK_OFFSET && far_field_applies(c, s, sizeof(T) == 4)) key.push_back('F');
   GreenCache& gc = c->gcache;
   GreenCache::Entry* hit = nullptr;
   // a captured solve builds its spectra on every replay (the spacing is only
@@ -449,9 +508,12 @@ GreenRef green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) {
   if (s.kind == GK_OFFSET && fft_has_plan(s.L[0])) {
     // the x pass takes the 8-corner difference of the primitive as it loads
     // its rows, so the difference array is never stored
-    {
+    CornerDiff cd{s.L[1], s.L[2], s.out_shift_z, s.zero_raw};
+    int zlo, zhi;
+    far_field_setup(c, s, fast, cd, zlo, zhi);
+    if (zlo <= zhi) {
       ProfScope ps(c, ST_GREEN_EVAL);
-      build_offset_primitive(c, s, g.as<double>(), fast, true);
+      build_offset_primitive(c, s, g.as<double>(), fast, true, zlo, zhi);
     }
     // the corner difference of a primitive that is even along an axis is odd
     // about L/2 - 1 there, and even where the primitive is odd
@@ -460,7 +522,6 @@ GreenRef green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) {
     int sym[3] = {0, 0, 0};
     if (mev >= 0) sym[mev] = -1;
     if (mod >= 0) sym[mod] = 1;
-    const CornerDiff cd{s.L[1], s.L[2], s.out_shift_z, s.zero_raw};
     spectrum_forward_full<T>(c, g.as<const double>(), sp.as<cpx<T>>(), d, &cd, sym, zr);
   } else {
     build_green_real(c, s, g.as<double>(), fast);

@@ -256,6 +256,9 @@ struct osc_ctx {
   // (deposit_tiled) where the mesh allows; off (OSC_TILED_DEPOSIT=0): the
   // cell-sorted deposit everywhere
   bool tiled_deposit = true;
+  // FP32 Green builds take far cells from the field kernel's expansion
+  // (far_field_setup); off (OSC_FAR_GREEN=0): every cell from the primitive
+  bool far_green = true;
   std::map<std::string, std::unique_ptr<osc::SolveGraph>> sgraphs;
   uint64_t graph_tick = 0;
   int64_t graph_builds = 0, graph_replays = 0;
