@@ -290,6 +290,23 @@ __device__ __forceinline__ double far_cell(const CornerDiff& cd, double u, doubl
   return cd.factor * (hu * hv * hw) *
          (f + (hu * hu * fuu + hv * hv * fvv + hw * hw * fww) * (1.0 / 24.0));
 }
+
+// far_cell for an FP32 x pass: the cell centre formed in FP64 (its position
+// is a large multiple of the edge), the expansion in FP32; K = factor times
+// the cell volume, e2 = the squared edges over 24
+__device__ __forceinline__ float far_cell_f(int icomp, float u, float v, float w, float K,
+                                            float e2u, float e2v, float e2w) {
+  const float r2 = u * u + v * v + w * w;
+  const float ir = rsqrtf(r2), ir2 = ir * ir;
+  const float xa = icomp == 1 ? u : (icomp == 2 ? v : w);
+  const float f = xa * ir2 * ir;
+  const float t15 = 15.0f * f * ir2 * ir2, t3 = 3.0f * f * ir2;
+  float fuu = t15 * u * u - t3, fvv = t15 * v * v - t3, fww = t15 * w * w - t3;
+  if (icomp == 1) fuu -= 2.0f * t3;
+  else if (icomp == 2) fvv -= 2.0f * t3;
+  else fww -= 2.0f * t3;
+  return K * (f + e2u * fuu + e2v * fvv + e2w * fww);
+}
 // element p of a corner-difference row, with the terms in the order of the
 // upstream loop (a left-to-right chain of adds and subtracts)
 __device__ __forceinline__ double diff_elem(const double* q, const R2CIO& a, const CornerDiff& cd,
@@ -326,17 +343,37 @@ __global__ void __launch_bounds__((XGeo<T, L>::NT))
       if (ka < 0) ka += cd.L2;
       if (kb < 0) kb += cd.L2;
       const double wa = cd.c0[2] + ka * cd.h[2], wb = cd.c0[2] + kb * cd.h[2];
-      f.load([&](int, int p) {
-        const double u = cd.c0[0] + p * cd.h[0];
-        T re = T(0), im = T(0);
-        if (oka && p < a.n_in)
-          re = fara ? (p < a.n_in - 1 ? T(far_cell(cd, u, va, wa)) : T(0))
-                    : T(diff_elem(pa, a, cd, ina, p));
-        if (okb && p < a.n_in)
-          im = farb ? (p < a.n_in - 1 ? T(far_cell(cd, u, vb, wb)) : T(0))
-                    : T(diff_elem(pb, a, cd, inb, p));
-        return mk(re, im);
-      });
+      if constexpr (sizeof(T) == 4) {
+        const float K = float(cd.factor * (cd.h[0] * cd.h[1] * cd.h[2]));
+        const float e2u = float(cd.h[0] * cd.h[0] / 24.0), e2v = float(cd.h[1] * cd.h[1] / 24.0),
+                    e2w = float(cd.h[2] * cd.h[2] / 24.0);
+        const float vaf = float(va), vbf = float(vb), waf = float(wa), wbf = float(wb);
+        f.load([&](int, int p) {
+          const float u = float(cd.c0[0] + p * cd.h[0]);
+          T re = T(0), im = T(0);
+          if (oka && p < a.n_in)
+            re = fara ? (p < a.n_in - 1 ? far_cell_f(cd.icomp, u, vaf, waf, K, e2u, e2v, e2w)
+                                        : T(0))
+                      : T(diff_elem(pa, a, cd, ina, p));
+          if (okb && p < a.n_in)
+            im = farb ? (p < a.n_in - 1 ? far_cell_f(cd.icomp, u, vbf, wbf, K, e2u, e2v, e2w)
+                                        : T(0))
+                      : T(diff_elem(pb, a, cd, inb, p));
+          return mk(re, im);
+        });
+      } else {
+        f.load([&](int, int p) {
+          const double u = cd.c0[0] + p * cd.h[0];
+          T re = T(0), im = T(0);
+          if (oka && p < a.n_in)
+            re = fara ? (p < a.n_in - 1 ? T(far_cell(cd, u, va, wa)) : T(0))
+                      : T(diff_elem(pa, a, cd, ina, p));
+          if (okb && p < a.n_in)
+            im = farb ? (p < a.n_in - 1 ? T(far_cell(cd, u, vb, wb)) : T(0))
+                      : T(diff_elem(pb, a, cd, inb, p));
+          return mk(re, im);
+        });
+      }
     } else {
       f.load([&](int, int p) {
         T re = (oka && p < a.n_in) ? T(diff_elem(pa, a, cd, ina, p)) : T(0);
