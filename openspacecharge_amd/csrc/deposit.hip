@@ -896,7 +896,7 @@ __global__ void __launch_bounds__(kThreads)
 //
 // The 8 flags and the 8 corner sums of a node are read together and the sums
 // of unflagged cells (memory never written this time) are selected away, so
-// a node costs one round of loads.  With `sparse` (far fewer particles than
+// a node costs one round of loads.  With `sparse` (fewer particles than
 // cells, most cells empty) the flags are read first and only the sums of
 // flagged cells are loaded: two rounds of loads, but the sums of the empty
 // cells, most of the 8 planes, are never fetched.  The accumulator starts at
@@ -1881,8 +1881,9 @@ void deposit_cic(osc_ctx* c, const DepositArgs& a, double* rho, SortedParticles*
   }
   {
     ProfScope ps(c, ST_DEP_NODES);
-    // at most one cell in 8 holds particles: load the sums of flagged cells only
-    const int sparse = n < ncell / 8 ? 1 : 0;
+    // fewer particles than cells (most cells empty): load the sums of flagged
+    // cells only
+    const int sparse = n < ncell ? 1 : 0;
     if (f32)
       hipLaunchKernelGGL(k_node<float>, dim3(nblk(ncell)), dim3(kThreads), 0, c->stream,
                          S.as<const float>(), occ.as<const uint8_t>(), nx, ny, nz, sparse,
