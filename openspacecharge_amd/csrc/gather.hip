@@ -495,9 +495,10 @@ __global__ void __launch_bounds__(kThreads)
   if (threadIdx.x == 0 && cnt) atomicAdd(nclamp, cnt);
 }
 
-// The cell-ordered gather of a sparse step (far fewer particles than nodes):
-// each particle reads its 8 nodes from the planar fields where they lie, so
-// that no node records of the whole mesh are packed for a few particles.
+// The cell-ordered gather of a step with fewer particles than twice the
+// nodes: each particle reads its 8 nodes from the planar fields where they
+// lie (neighbours in cell order share their lines), so that no node records
+// of the whole mesh are packed.
 // Cells and weights as in k_interp_cells (FP64: from the record's
 // coordinates) and k_interp_cells32 (F32: the deposit's key and FP32
 // weights; off the mesh, the clamp rule from the coordinates), the same 8
@@ -925,9 +926,10 @@ void interpolate_sorted(osc_ctx* c, const InterpArgs& a, const double* ef, const
     OSC_HIP(hipGetLastError());
     return;
   }
-  if (sp.n < nt / 8 && !packed) {
-    // few particles for the mesh: node values from the planar fields, no
-    // node records packed
+  if (!packed && sp.n < 2 * nt) {
+    // fewer particles than twice the nodes: node values from the planar
+    // fields, no node records packed (at 256^3 the pack alone costs about
+    // what the planar reads of 10^7 particles do)
     OSC_REQUIRE(!sp.f32 || (x && y && z), OSC_ERR_ARG, "FP32 gather: coordinates needed");
     const int64_t n = sp.n;
     if (n <= 0) return;

@@ -1,12 +1,14 @@
-"""The fused step with far fewer particles than nodes (n < nodes/8) on the
-cell-sorted deposit, taken by meshes whose tiles do not fit (cubic meshes
-above 161^3 nodes) and by contexts created with OSC_TILED_DEPOSIT=0: the node sums load
-only the corner sums of occupied cells, and the gather reads the planar
-fields instead of packing node records of the whole mesh.
+"""The fused step with fewer particles than cells on the cell-sorted
+deposit, taken by meshes whose tiles do not fit (cubic meshes above 161^3
+nodes) and by contexts created with OSC_TILED_DEPOSIT=0: the node sums load
+only the corner sums of occupied cells, and below two particles per node the
+gather reads the planar fields instead of packing node records of the whole
+mesh.
 
 FP64: rho, the fields and the particles' E and B bit-identical to the three
-separate calls (whose interpolate_field reads the planar fields for so few
-points, in particle order), rho within 1e-14 of max rho of the oracle's
+separate calls (whose interpolate_field reads the planar fields in particle
+order below nodes/8 points, and packed node records in the deposit's cell
+order above: dense192), rho within 1e-14 of max rho of the oracle's
 deposit and of the tiled deposit; a fixed mesh drops and clamps off-mesh
 particles as the separate calls do.  FP32: E within 1e-5 of FP64 (per
 component, relative to the mesh maximum), two runs bit-identical (the clamp
@@ -50,7 +52,7 @@ def relmax(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
-# nodes / 8 above n in every case
+# n below the cells in every case (below nodes / 8 but for dense192)
 CASES = {
     "resized": dict(nhi=(64, 48, 80), n=20000, sig=(1e-3, 1e-3, 2e-3)),
     "charges": dict(nhi=(48, 40, 56), n=8000, sig=(1e-3, 0.6e-3, 2e-3), q=True),
@@ -58,12 +60,13 @@ CASES = {
     "cathode": dict(nhi=(40, 36, 64), n=10000, sig=(1e-3, 1e-3, 1e-3), ctr=(0, 0, 6e-3),
                     cath=True),
     "mesh192": dict(nhi=(192, 192, 192), n=100000, sig=(1e-3, 1e-3, 2e-3)),
+    "dense192": dict(nhi=(192, 192, 192), n=2_000_000, sig=(1e-3, 1e-3, 2e-3)),
 }
 
 
 def _bunch(case):
     s = CASES[case]
-    assert s["n"] < np.prod(s["nhi"]) // 8
+    assert s["n"] < np.prod(s["nhi"]) // (1 if case == "dense192" else 8)
     x, y, z = gaussian_bunch(s["n"], s["sig"], seed=sorted(CASES).index(case) + 70,
                              center=s.get("ctr", (0, 0, 0)))
     q = None
@@ -98,7 +101,7 @@ def test_sparse_step_fp64_equals_separate_calls(ctxs, oracle, case):
     x, y, z, q = _bunch(case)
     print(f"case {case}", flush=True)
     # 192^3 takes the cell-sorted deposit in a default context
-    ctx = ctxs[(osc.OSC_PREC_FP64, case == "mesh192")]
+    ctx = ctxs[(osc.OSC_PREC_FP64, case in ("mesh192", "dense192"))]
     m2, E2, B2 = _step(ctx, case, x, y, z, q)
     m1, kw = _mesh(case)
     osc.deposit_particles(x, y, z, m1, qa=q, total_charge=None if q is not None else 1e-9,
@@ -118,12 +121,12 @@ def test_sparse_step_fp64_equals_separate_calls(ctxs, oracle, case):
         assert relmax(m2.rho, d["rho"]) < 1e-14
 
 
-@pytest.mark.parametrize("case", ["resized", "fixed_offmesh", "cathode", "mesh192"])
+@pytest.mark.parametrize("case", ["resized", "fixed_offmesh", "cathode", "mesh192", "dense192"])
 def test_sparse_step_fp32_within_1e5(ctxs, case):
     x, y, z, q = _bunch(case)
     print(f"case {case}", flush=True)
     m64, E64, B64 = _step(ctxs[(osc.OSC_PREC_FP64, False)], case, x, y, z, q)
-    c32 = ctxs[(osc.OSC_PREC_FP32, case == "mesh192")]
+    c32 = ctxs[(osc.OSC_PREC_FP32, case in ("mesh192", "dense192"))]
     m, E, B = _step(c32, case, x, y, z, q)
     assert np.array_equal(m.delta, m64.delta)
     assert relmax(m.rho, m64.rho) < 1e-6
