@@ -241,6 +241,63 @@ __global__ void __launch_bounds__(256)
   G[ii + int64_t(s.L[0]) * (jj + int64_t(s.L[1]) * kk)] = res;
 }
 
+// The pipe Green spectrum from its separable form: G_t(i, j, k) =
+// C sum_{m,n} A[i,m] B[j,n] Z[k,m,n] with A = i^m cos(m u pi/a) (cu),
+// B = j^n cos(n v pi/b) (cv), Z = zfun_mn(w) / kap_mn (zf, kap) and
+// C = 2 pi / (a b), so that its DFT is C sum_{m,n} A^[kx,m] B^[ky,n]
+// Z^[kz,m,n] with 1-D DFTs of the profiles.
+// k_pipe_dft: out[k P + q] = sum_j w(j, q) exp(-2 pi i k j / L) for k < K,
+// q < P, with the length's twiddle table; w = s^(q+1) prof (s = sign, the
+// mode's power of i or j), or prof / kap[q] when kap is given.  One wave per
+// output: lane l sums j = l, l + 64, ..., then the wave adds the lanes.
+__global__ void __launch_bounds__(64)
+    k_pipe_dft(const double* __restrict__ prof, int P, int L, int sign,
+               const double* __restrict__ kap, const cpx<double>* __restrict__ tw,
+               cpx<double>* __restrict__ out) {
+  const int k = blockIdx.x, q = blockIdx.y, lane = threadIdx.x;
+  const double sq = (sign < 0 && (q % 2) == 0) ? -1.0 : 1.0;  // (-1)^(q+1)
+  double re = 0.0, im = 0.0;
+  for (int j = lane; j < L; j += 64) {
+    const double v = kap ? prof[j * P + q] / kap[q] : sq * prof[j * P + q];
+    const cpx<double> w = tw[(k * j) % L];
+    re = re + v * w.x;
+    im = im + v * w.y;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    re = re + __shfl_xor(re, o, 64);
+    im = im + __shfl_xor(im, o, 64);
+  }
+  if (lane == 0) out[k * P + q] = mk(re, im);
+}
+
+// k_pipe_spec: one (ky, kz) row of the half spectrum (KXP entries, zero past
+// KX): T[m] = C sum_n B^[ky,n] Z^[kz,m,n], then sum_m A^[kx,m] T[m]
+__global__ void __launch_bounds__(256)
+    k_pipe_spec(const cpx<double>* __restrict__ ax, const cpx<double>* __restrict__ by,
+                const cpx<double>* __restrict__ cz, int KX, int KXP, int L1, double C,
+                cpx<double>* __restrict__ spec) {
+  // one wave per row, four rows per workgroup
+  const int ky = blockIdx.x * 4 + (threadIdx.x >> 6), kz = blockIdx.y;
+  if (ky >= L1) return;
+  cpx<double> T[5];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    cpx<double> acc = mk(0.0, 0.0);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) acc = add(acc, mul(by[ky * 5 + n], cz[kz * 25 + m * 5 + n]));
+    T[m] = mk(acc.x * C, acc.y * C);
+  }
+  cpx<double>* row = spec + (int64_t(kz) * L1 + ky) * KXP;
+  for (int kx = threadIdx.x & 63; kx < KXP; kx += 64) {
+    cpx<double> v = mk(0.0, 0.0);
+    if (kx < KX)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) v = add(v, mul(ax[kx * 5 + m], T[m]));
+    row[kx] = v;
+  }
+}
+
 __global__ void k_convert(const cpx<double>* __restrict__ in, cpx<float>* __restrict__ out,
                           int64_t n) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
@@ -323,6 +380,45 @@ static void build_offset_primitive(osc_ctx* c, const GreenSpec& s, double* F, bo
   launch(k_prim_offset, gp, dim3(64), 0, c->stream, F, s.L[0], s.L[1], s.L[2], dx, dy,
                      dz, umin, vmin, wmin, factor, s.icomp, mev, mod, half, zlo, zhi, zbase,
                      md, s.gamma, int(md && s.out_shift_z != 0));
+  OSC_HIP(hipGetLastError());
+}
+
+// The half spectrum of one pipe Green term (s.kind GK_PIPE) built from the
+// separable form (k_pipe_dft, k_pipe_spec): no real-space array and no 3-D
+// transform.  Agrees with the transformed real array (build_green_real, then
+// spectrum_forward_full) to rounding.
+void pipe_spectrum(osc_ctx* c, const GreenSpec& s, const SpecDims& d, cpx<double>* spec) {
+  const int si = (s.term & 2) ? -1 : 1, sj = (s.term & 1) ? -1 : 1;
+  const int L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
+  const int lmax = std::max(L0, std::max(L1, L2));
+  DevBuf tab(c->pool, sizeof(double) * (5 * (L0 + L1) + 25 * L2 + 25) +
+                          sizeof(cpx<double>) * (5 * (d.KX + L1) + 25 * L2));
+  // the complex profiles first (16-byte aligned), then the real tables
+  cpx<double>* ax = tab.as<cpx<double>>();
+  cpx<double>* by = ax + 5 * d.KX;
+  cpx<double>* cz = by + 5 * L1;
+  double* cu = reinterpret_cast<double*>(cz + 25 * L2);
+  double* cv = cu + 5 * L0;
+  double* zf = cv + 5 * L1;
+  double* kap = zf + 25 * L2;
+  ProfScope ps(c, ST_GREEN_EVAL);
+  launch(k_pipe_tables, dim3((lmax + 63) / 64), dim3(64), 0, c->stream, s, si, sj, cu, cv, zf,
+         kap);
+  OSC_HIP(hipGetLastError());
+  launch(k_pipe_dft, dim3(unsigned(d.KX), 5), dim3(64), 0, c->stream,
+         static_cast<const double*>(cu), 5, L0, si, static_cast<const double*>(nullptr),
+         twiddles<double>(c, L0), ax);
+  launch(k_pipe_dft, dim3(unsigned(L1), 5), dim3(64), 0, c->stream,
+         static_cast<const double*>(cv), 5, L1, sj, static_cast<const double*>(nullptr),
+         twiddles<double>(c, L1), by);
+  launch(k_pipe_dft, dim3(unsigned(L2), 25), dim3(64), 0, c->stream,
+         static_cast<const double*>(zf), 25, L2, 1, static_cast<const double*>(kap),
+         twiddles<double>(c, L2), cz);
+  OSC_HIP(hipGetLastError());
+  const double C = 2.0 * kPi * (1.0 / s.a) * (1.0 / s.b);
+  launch(k_pipe_spec, dim3(unsigned((L1 + 3) / 4), unsigned(L2)), dim3(256), 0, c->stream,
+         static_cast<const cpx<double>*>(ax), static_cast<const cpx<double>*>(by),
+         static_cast<const cpx<double>*>(cz), d.KX, d.KXP, L1, C, spec);
   OSC_HIP(hipGetLastError());
 }
 

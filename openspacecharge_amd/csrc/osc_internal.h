@@ -393,6 +393,8 @@ bool is_pageable(const void* p);
 
 // ---- green.hip --------------------------------------------------------------
 void build_green_real(osc_ctx* c, const GreenSpec& s, double* g, bool fast = false);  // (L0,L1,L2) real
+// the half spectrum of one pipe Green term from its separable form
+void pipe_spectrum(osc_ctx* c, const GreenSpec& s, const SpecDims& d, cpx<double>* spec);
 // spectrum of a Green function in precision T (z-real when the z length has a
 // compile-time plan); cached by spec
 template <typename T>

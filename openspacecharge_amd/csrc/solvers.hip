@@ -351,8 +351,9 @@ int rectpipe(osc_ctx* c, const osc_mesh& m, int direct, const double* rho, doubl
   return pipe_t<double>(c, m, direct, rho, phi, ef, bf);
 }
 
-// osc_getgrnpipe (core:632-715): four real arrays on the pipe grid, each
-// transformed to a half spectrum (upstream signs are applied only on file I/O)
+// osc_getgrnpipe (core:632-715): the half spectra of the four real arrays on
+// the pipe grid, built from their separable form (pipe_spectrum; upstream
+// signs are applied only on file I/O)
 void pipe_build(osc_ctx* c, double gamma, double a, double b, const double delta[3],
                 const double umin[3], const int npad[3]) {
   PipeState& p = c->pipe;
@@ -370,13 +371,10 @@ void pipe_build(osc_ctx* c, double gamma, double a, double b, const double delta
   s.a = a;
   s.b = b;
   const SpecDims d = SpecDims::of(p.L);
-  const int64_t N = int64_t(p.L[0]) * p.L[1] * p.L[2];
-  DevBuf g(c->pool, sizeof(double) * N);
   for (int t = 0; t < 4; ++t) {
     s.term = t;
-    build_green_real(c, s, g.as<double>());
     if (!p.spec64[t].p) p.spec64[t] = DevBuf(c->pool, sizeof(cpx<double>) * d.size());
-    spectrum_forward_full(c, g.as<const double>(), p.spec64[t].as<cpx<double>>(), d);
+    pipe_spectrum(c, s, d, p.spec64[t].as<cpx<double>>());
     p.spec32[t].reset();
   }
   p.built = true;
