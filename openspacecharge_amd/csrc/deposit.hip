@@ -1414,8 +1414,23 @@ __global__ void k_tile_nodes(const unsigned long long* __restrict__ part,
           const int t = tx + p.nt[0] * (ty + p.nt[1] * tz);
           const int local = (i - (tx << p.tb[0])) +
                             p.tn[0] * ((j - (ty << p.tb[1])) + p.tn[1] * (k - (tz << p.tb[2])));
-          for (uint32_t c = tp->wl[t]; c < tp->wl[t + 1]; ++c) {
-            const unsigned long long* blk = part + int64_t(c) * p.nodes * W + local;
+          // four chunk blocks at a time, their loads in flight together
+          // (integer sums: the order does not matter)
+          const uint32_t c1 = tp->wl[t + 1];
+          uint32_t c = tp->wl[t];
+          const int64_t cs = int64_t(p.nodes) * W;
+          for (; c + 4 <= c1; c += 4) {
+            const unsigned long long* blk = part + int64_t(c) * cs + local;
+            const unsigned long long h0 = blk[0], h1 = blk[cs], h2 = blk[2 * cs], h3 = blk[3 * cs];
+            H += (h0 + h1) + (h2 + h3);
+            if (!F32) {
+              const unsigned long long l0 = blk[p.nodes], l1 = blk[cs + p.nodes],
+                                       l2 = blk[2 * cs + p.nodes], l3 = blk[3 * cs + p.nodes];
+              L += (l0 + l1) + (l2 + l3);
+            }
+          }
+          for (; c < c1; ++c) {
+            const unsigned long long* blk = part + int64_t(c) * cs + local;
             H += blk[0];
             if (!F32) L += blk[p.nodes];
           }
