@@ -749,10 +749,18 @@ __global__ void __launch_bounds__((ZGeo<T, L>::NT))
     });
     __syncthreads();  // the previous transform's last reads of lds
     fg.run(lds, tw);
+    // the term's sign: a multiplication by +1 is the identity (uniform
+    // branch; the same values as sgn * q).  FP32 only: in FP64 the second
+    // copy of the loop costs registers (124 -> 138 VGPRs, a wave per SIMD)
+    if (sizeof(T) == 4 && tm.sgn == T(1)) {
 #pragma unroll
-    for (int i = 0; i < FF::E; ++i) {
-      const cpx<T> q = mul(fr.v[i], fg.v[i]);
-      fg.v[i] = mk(tm.sgn * q.x, tm.sgn * q.y);
+      for (int i = 0; i < FF::E; ++i) fg.v[i] = mul(fr.v[i], fg.v[i]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < FF::E; ++i) {
+        const cpx<T> q = mul(fr.v[i], fg.v[i]);
+        fg.v[i] = mk(tm.sgn * q.x, tm.sgn * q.y);
+      }
     }
     __syncthreads();  // the transform's last reads of lds
     fg.dump(lds);
