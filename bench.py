@@ -68,6 +68,9 @@ def parse():
                    help="bunches of the batched mixed-grid leg (0 skips it)")
     p.add_argument("--served-requests", type=int, default=64,
                    help="requests of the served mixed stream (0 skips it)")
+    p.add_argument("--stream-reps", type=int, default=3,
+                   help="timed runs of each stream configuration on one warmed server, "
+                        "its Green caches cleared before each; the median wall is reported")
     p.add_argument("--poisson-requests", type=int, default=1000,
                    help="requests of each Poisson-arrival run of the served mix (three loads)")
     return p.parse_args()
@@ -314,28 +317,32 @@ def make_stream(kind, count, seed):
     return reqs
 
 
-def run_stream(reqs, parts, nworkers, precision, green_cache, sequential, rate=None, seed=1,
-               warm=None):
-    """Runs the requests through a Server; returns (wall s, per-request
-    records, outputs, server counters of the timed stream).  sequential: each
-    request waits for the previous one (plain per-request solving).  rate:
-    Poisson arrivals at this many requests/s (otherwise all at once).  warm:
-    (requests, bunches) of another seed run first on the same server, untimed,
-    so that the timed stream finds the device buffer pools of a running server
-    (its meshes differ, so it leaves no Green spectra the timed stream could
-    reuse)."""
+def run_stream(reqs, parts, nworkers, precision, green_cache, sequential, warm, reps=1):
+    """Times the requests `reps` times on one Server, after an untimed run of
+    `warm` = (requests, bunches) of another seed, so that the timed runs find
+    the device buffer pools of a running server (its meshes differ, so it
+    leaves no Green spectra the timed stream could reuse).  Before each timed
+    run the server's Green caches are cleared (their buffers stay pooled: no
+    device memory is freed), so every run builds every spectrum, as a stream
+    of new bunches does.  sequential: each request waits for the previous one
+    (plain per-request solving).  Returns the median wall (s), every wall, and
+    the last run's per-request records, outputs and server counters."""
     import openspacecharge_amd as osc
     srv = osc.Server(nworkers=nworkers, green_cache=green_cache)
-    if warm is not None:
-        _drive(srv, warm[0], warm[1], precision, sequential)
-    st0 = srv.stats()
-    wall, recs, outs = _drive(srv, reqs, parts, precision, sequential, rate, seed)
-    st1 = srv.stats()
+    _drive(srv, warm[0], warm[1], precision, sequential)
+    walls, recs, outs = [], None, None
+    for _ in range(max(1, reps)):
+        srv.clear_caches()
+        outs = None
+        st0 = srv.stats()
+        wall, recs, outs = _drive(srv, reqs, parts, precision, sequential)
+        st1 = srv.stats()
+        walls.append(wall)
     srv.close()
     st = {k: st1[k] - st0[k] for k in ("done", "affine_starts", "bypasses", "green_hits",
                                        "green_misses", "trims", "allocs", "alloc_us")}
     st["max_running"] = st1["max_running"]
-    return wall, recs, outs, st
+    return sorted(walls)[len(walls) // 2], walls, recs, outs, st
 
 
 def long_stream(srv, reqs, parts, nreq, rate, precision, seed=5):
@@ -432,12 +439,15 @@ def stream_legs(args):
         wparts = [gpu_bunch(r["n"], r["sigma"], r["seed"], r["center"], r["zpos"]) for r in wreqs]
         warm = (wreqs, wparts)
         torch.cuda.synchronize()
+        reps = args.stream_reps
         # plain FP64 per request: one at a time, the Green function rebuilt for every request
-        w64, _, o64, _ = run_stream(reqs, parts, 1, osc.OSC_PREC_FP64, False, True, warm=warm)
+        w64, l64, _, o64, _ = run_stream(reqs, parts, 1, osc.OSC_PREC_FP64, False, True, warm,
+                                         reps)
         # served: concurrent workers, Green spectra kept resident
-        ws64, _, _, st64 = run_stream(reqs, parts, 4, osc.OSC_PREC_FP64, True, False, warm=warm)
-        ws32, recs32, o32, st32 = run_stream(reqs, parts, 4, osc.OSC_PREC_FP32, True, False,
-                                             warm=warm)
+        ws64, ls64, _, _, st64 = run_stream(reqs, parts, 4, osc.OSC_PREC_FP64, True, False, warm,
+                                            reps)
+        ws32, ls32, recs32, o32, st32 = run_stream(reqs, parts, 4, osc.OSC_PREC_FP32, True, False,
+                                                   warm, reps)
         err = 0.0
         for a, b in zip(o32, o64):
             a, b = a.cpu().numpy(), b.cpu().numpy()
@@ -450,6 +460,10 @@ def stream_legs(args):
                "fp32_served_vs_fp64_per_request": round(w64 / ws32, 3),
                "target_3x": "pass" if w64 / ws32 >= 3.0 else "fail",
                "max_rel_err_fp32_vs_fp64": err,
+               "repetitions": max(1, reps),
+               "walls_ms": {"fp64_per_request": [round(w * 1e3, 2) for w in l64],
+                            "fp64_served": [round(w * 1e3, 2) for w in ls64],
+                            "fp32_served": [round(w * 1e3, 2) for w in ls32]},
                "server_fp64": st64, "server_fp32": st32}
         if kind == "served":
             out["mix"] = "70% free / 20% cathode / 10% pipe, grid edge 32..256 (2^a 3^b), 1e5-2e7 particles"

@@ -76,7 +76,10 @@ int osc_ctx_destroy(osc_ctx* ctx);
 int osc_ctx_set_stream(osc_ctx* ctx, void* hip_stream);
 int osc_ctx_set_async(osc_ctx* ctx, int async);
 int osc_ctx_set_precision(osc_ctx* ctx, int precision);
-/* Green-spectrum cache: enable (default 1) and byte budget (default 8 GiB) */
+/* Green-spectrum cache: enable (default 1) and byte budget (default 8 GiB).
+ * A spectrum is kept from the second solve that misses on its mesh, spacing
+ * and solver on; a first-seen one is built into a pooled buffer that later
+ * solves reuse (no device allocation per new bunch shape). */
 int osc_ctx_set_green_cache(osc_ctx* ctx, int enable, int64_t max_bytes);
 /* Fused steps (osc_space_charge_step) with a resized (bunch-following) mesh
  * of at most 128^3 nodes and no offset record their grid part -- Green
@@ -300,6 +303,12 @@ int osc_server_submit(osc_server* server, const osc_request* req, int64_t* ticke
  * deposit, points clamped by the gather.  Returns the request's status. */
 int osc_server_wait(osc_server* server, int64_t ticket, double* times_ms, int64_t* counts);
 int osc_server_drain(osc_server* server);
+/* waits until no request is in flight, then drops every worker's cached Green
+ * spectra and pipe Green functions into its buffer pool (a service after its
+ * lattice or bunch set changes; a benchmark repeating a stream of new bunches
+ * on a warm server).  No device memory is freed.  No upstream counterpart:
+ * upstream keeps one set of pipe Green functions and no spectrum cache. */
+int osc_server_clear_caches(osc_server* server);
 /* counters since creation, the first n of: requests done, most requests
  * running at once, requests started on a worker that served the same geometry
  * before, requests started ahead of an older one that did not fit, Green

@@ -151,6 +151,7 @@ SIGNATURES = {
     "osc_server_submit": (C.c_int, [_vp, C.c_void_p, C.POINTER(_i64)]),
     "osc_server_wait": (C.c_int, [_vp, _i64, _dp, C.POINTER(_i64)]),
     "osc_server_drain": (C.c_int, [_vp]),
+    "osc_server_clear_caches": (C.c_int, [_vp]),
     "osc_server_stats": (C.c_int, [_vp, C.POINTER(_i64), C.c_int]),
     "osc_server_last_error": (C.c_char_p, [_vp]),
     "osc_server_destroy": (C.c_int, [_vp]),

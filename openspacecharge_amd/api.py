@@ -588,6 +588,14 @@ class Server:
     def drain(self) -> None:
         self.lib.osc_server_drain(self.h)
 
+    def clear_caches(self) -> None:
+        """Waits until no request is in flight, then returns every worker's
+        cached Green spectra and pipe Green functions to its buffer pool
+        (osc_server_clear_caches); no device memory is freed."""
+        rc = self.lib.osc_server_clear_caches(self.h)
+        if rc != K.OSC_OK:
+            raise K.OSCError(rc, "osc_server_clear_caches failed")
+
     STATS = ("done", "max_running", "affine_starts", "bypasses", "green_hits", "green_misses",
              "held_bytes", "budget_bytes", "trims", "trimmed_bytes", "allocs", "alloc_us")
 

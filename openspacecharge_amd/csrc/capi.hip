@@ -110,6 +110,7 @@ OSC_API int osc_ctx_create(int device, osc_ctx** out) {
   c->stream = c->own;
   if (const char* td = std::getenv("OSC_TILED_DEPOSIT")) c->tiled_deposit = td[0] != '0';
   if (const char* fg = std::getenv("OSC_FAR_GREEN")) c->far_green = fg[0] != '0';
+  if (const char* ga = std::getenv("OSC_GREEN_ADMIT")) c->gcache.admit_second = ga[0] != '0';
   // the device address of the no-allocation gather's scratch on this device
   const int rc = noalloc_setup(device, &c->last_error);
   if (rc != OSC_OK) {

@@ -625,7 +625,7 @@ GreenRef green_spectrum(osc_ctx* c, const GreenSpec& s, DevBuf& keepalive) {
   }
   g.reset();
   ref.g = sp.p;
-  if (!use_cache) {
+  if (!use_cache || (!hit && !gc.admit(key))) {
     keepalive = std::move(sp);
     return ref;
   }

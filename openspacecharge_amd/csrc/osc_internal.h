@@ -3,6 +3,7 @@
 #include <list>
 #include <map>
 #include <memory>
+#include <unordered_set>
 
 #include "fft_launch.h"
 
@@ -79,8 +80,24 @@ class GreenCache {
   void unpin() {
     if (--pin_depth == 0) pin_from = ~uint64_t(0);
   }
+  // admission: a spectrum is kept from its key's second miss on.  One whose
+  // key is new (a bunch whose spacing does not recur) is built into a pooled
+  // buffer that goes back to the pool after its solve, where the next request
+  // of that size finds it, instead of a new device allocation held by the
+  // cache; `seen` holds the keys missed once (at most kSeenMax, then
+  // cleared).  admit_second = false keeps every spectrum built
+  static constexpr size_t kSeenMax = 4096;
+  bool admit_second = true;
+  std::unordered_set<std::string> seen;
+  bool admit(const std::string& key) {
+    if (!admit_second || seen.erase(key)) return true;
+    if (seen.size() >= kSeenMax) seen.clear();
+    seen.insert(key);
+    return false;
+  }
   void clear() {
     map.clear();
+    seen.clear();
     bytes = 0;
   }
   void evict_to(size_t budget);

@@ -357,6 +357,20 @@ OSC_API int osc_server_drain(osc_server* s) {
   return OSC_OK;
 }
 
+// Once no request is in flight (the workers' streams then idle), every
+// worker's cached Green spectra, its keys seen once and its pipe Green
+// functions go back to its buffer pool: no device memory is freed.
+OSC_API int osc_server_clear_caches(osc_server* s) {
+  if (!s) return OSC_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [&] { return s->in_flight == 0; });
+  for (auto* c : s->ctxs) {
+    c->gcache.clear();
+    c->pipe = osc::PipeState{};
+  }
+  return OSC_OK;
+}
+
 OSC_API int osc_server_stats(osc_server* s, int64_t* out, int n) {
   if (!s || !out || n < 1) return OSC_ERR_ARG;
   std::lock_guard<std::mutex> lk(s->mu);

@@ -128,13 +128,36 @@ def test_server_is_work_conserving():
     st = srv.stats()
     srv.close()
     assert st["done"] == 12 and st["max_running"] >= 2, st
-    # E x, y, z spectra: at most one build of each per worker
-    assert st["green_misses"] <= 3 * 4 and st["green_hits"] >= 3 * (12 - 4), st
+    # E x, y, z spectra: at most two builds of each per worker (a spectrum is
+    # cached from its second miss on)
+    assert st["green_misses"] <= 3 * 2 * 4 and st["green_hits"] >= 3 * (12 - 2 * 4), st
     ref_srv = osc.Server(nworkers=1)
     ref = _run(ref_srv, reqs[:1], osc.OSC_PREC_FP64, concurrent=False)
     ref_srv.close()
     for E, B in res:
         assert np.array_equal(E, ref[0][0]) and np.array_equal(B, ref[0][1])
+
+
+def test_server_green_admission_and_clear():
+    """A geometry's Green spectra are cached from its second miss on (the
+    first build goes back to the buffer pool), hit from the third request;
+    osc_server_clear_caches drops them and the keys seen, so the next request
+    of that geometry builds again.  Results bit-identical throughout."""
+    (t, kw) = _mix()[1]
+    kw = dict(kw, mesh_min=(-2e-3, -2e-3, -8e-3), mesh_max=(2e-3, 2e-3, 8e-3))
+    srv = osc.Server(nworkers=1)
+    counts, res = [], []
+    for k in range(5):
+        if k == 3:
+            srv.clear_caches()
+        res += _run(srv, [(t, kw)], osc.OSC_PREC_FP64, concurrent=False)
+        st = srv.stats()
+        counts.append((st["green_misses"], st["green_hits"]))
+    srv.close()
+    m = counts[0][0]
+    assert m > 0 and counts == [(m, 0), (2 * m, 0), (2 * m, m), (3 * m, m), (4 * m, m)], counts
+    for E, B in res[1:]:
+        assert np.array_equal(E, res[0][0]) and np.array_equal(B, res[0][1])
 
 
 def test_server_reports_request_errors():
