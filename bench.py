@@ -68,6 +68,13 @@ def parse():
                    help="bunches of the batched mixed-grid leg (0 skips it)")
     p.add_argument("--served-requests", type=int, default=64,
                    help="requests of the served mixed stream (0 skips it)")
+    p.add_argument("--batch-workers", type=int, default=8,
+                   help="request-loop workers of the batch stream's served runs (meshes up to "
+                        "128^3: 8 workers 3.12x, 4 workers 2.94-2.96x)")
+    p.add_argument("--served-workers", type=int, default=4,
+                   help="request-loop workers of the served mix (meshes up to 256^3, 2e7 "
+                        "particles: with 8 the pools' device allocations stalled for tens of ms "
+                        "each)")
     p.add_argument("--stream-reps", type=int, default=3,
                    help="timed runs of each stream configuration on one warmed server, "
                         "its Green caches cleared before each; the median wall is reported")
@@ -440,13 +447,14 @@ def stream_legs(args):
         warm = (wreqs, wparts)
         torch.cuda.synchronize()
         reps = args.stream_reps
+        nw = args.batch_workers if kind == "batch" else args.served_workers
         # plain FP64 per request: one at a time, the Green function rebuilt for every request
         w64, l64, _, o64, _ = run_stream(reqs, parts, 1, osc.OSC_PREC_FP64, False, True, warm,
                                          reps)
         # served: concurrent workers, Green spectra kept resident
-        ws64, ls64, _, _, st64 = run_stream(reqs, parts, 4, osc.OSC_PREC_FP64, True, False, warm,
+        ws64, ls64, _, _, st64 = run_stream(reqs, parts, nw, osc.OSC_PREC_FP64, True, False, warm,
                                             reps)
-        ws32, ls32, recs32, o32, st32 = run_stream(reqs, parts, 4, osc.OSC_PREC_FP32, True, False,
+        ws32, ls32, recs32, o32, st32 = run_stream(reqs, parts, nw, osc.OSC_PREC_FP32, True, False,
                                                    warm, reps)
         err = 0.0
         for a, b in zip(o32, o64):
@@ -460,7 +468,7 @@ def stream_legs(args):
                "fp32_served_vs_fp64_per_request": round(w64 / ws32, 3),
                "target_3x": "pass" if w64 / ws32 >= 3.0 else "fail",
                "max_rel_err_fp32_vs_fp64": err,
-               "repetitions": max(1, reps),
+               "repetitions": max(1, reps), "workers": nw,
                "walls_ms": {"fp64_per_request": [round(w * 1e3, 2) for w in l64],
                             "fp64_served": [round(w * 1e3, 2) for w in ls64],
                             "fp32_served": [round(w * 1e3, 2) for w in ls32]},
@@ -473,7 +481,7 @@ def stream_legs(args):
                 # one running server for the three loads, as a service would be:
                 # warmed by the request set and an untimed Poisson run, so that
                 # its buffer pools hold the size classes of the stream
-                psrv = osc.Server(nworkers=4, green_cache=True)
+                psrv = osc.Server(nworkers=nw, green_cache=True)
                 _drive(psrv, reqs, parts, osc.OSC_PREC_FP32, False)
                 long_stream(psrv, reqs, parts, args.poisson_requests // 2, 0.5 * cap,
                             osc.OSC_PREC_FP32, seed=4)
