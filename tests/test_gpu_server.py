@@ -160,6 +160,49 @@ def test_server_green_admission_and_clear():
         assert np.array_equal(E, res[0][0]) and np.array_equal(B, res[0][1])
 
 
+def _new_shape_allocs(admit):
+    """Device allocations of a 1-worker server after each of eight FP32
+    requests on a 48^3 mesh that follows its bunch, the bunch length changing
+    every time (a new Green key per request)."""
+    import os
+    import torch
+    dev = torch.device("cuda:0")
+    old = os.environ.get("OSC_GREEN_ADMIT")
+    os.environ["OSC_GREEN_ADMIT"] = "1" if admit else "0"
+    try:
+        srv = osc.Server(nworkers=1)  # contexts read the switch at creation
+    finally:
+        if old is None:
+            del os.environ["OSC_GREEN_ADMIT"]
+        else:
+            os.environ["OSC_GREEN_ADMIT"] = old
+    allocs, res = [], []
+    for k in range(8):
+        x, y, z = gaussian_bunch(100_000, (1e-3, 1e-3, 1e-3 * (0.5 + 0.25 * k)), 300 + k)
+        t = [torch.from_numpy(a).to(dev) for a in (x, y, z)]
+        kw = dict(nhi=(48, 48, 48), gamma=GAMMA, total_charge=1e-9, solver="free")
+        res += _run(srv, [(t, kw)], osc.OSC_PREC_FP32, concurrent=False)
+        allocs.append(srv.stats()["allocs"])
+    st = srv.stats()
+    srv.close()
+    assert st["green_hits"] == 0 and st["green_misses"] == 3 * 8, st
+    return allocs, res
+
+
+def test_server_new_bunch_shapes_allocate_nothing():
+    """Spectra seen once are built into pooled buffers: once the first
+    requests have filled the worker's pool, a request whose spacing is new
+    allocates no device memory, where keeping every spectrum allocates for
+    each one.  Results equal either way."""
+    allocs, res = _new_shape_allocs(True)
+    keep, res_keep = _new_shape_allocs(False)
+    print("admit", allocs, "keep all", keep)
+    assert allocs[-1] == allocs[2], allocs
+    assert keep[-1] - keep[2] >= 3 * 5, keep
+    for (E, B), (Ek, Bk) in zip(res, res_keep):
+        assert np.array_equal(E, Ek) and np.array_equal(B, Bk)
+
+
 def test_server_reports_request_errors():
     import torch
     (t, kw) = _mix()[3]  # the pipe request
